@@ -1,0 +1,211 @@
+"""Benchmark of the plain tissue stepper on one MI355X.
+
+Workload (the 1024x1024 headline configuration): Fenton-Karma 3V, FP32,
+1024x1024 cells, dx = 0.025 cm, D = 0.001 cm^2/ms, dt = 0.02 ms, no-flux
+boundaries, 200 seeded circular holes (radii U[3,12] cells), u ~ U[0,1) with
+v = w = 1, tip tracking every 100 steps.  A "step" is one explicit-Euler time
+step of the whole tissue; the run is --steps such steps after --warmup.
+
+Printed: one JSON line with Gcell-steps/s, the roofline of the dominant kernel
+(timed live with HIP events on the stream it runs on) and the CPU baseline
+(the oracle's OpenMP loop of the same update on this host).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E peak
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--holes", type=int, default=200)
+    ap.add_argument("--model", choices=["fk", "2v"], default="fk")
+    ap.add_argument("--tip-every", type=int, default=100)
+    ap.add_argument("--cfg", default="auto", help="auto or K,rows,waves")
+    ap.add_argument("--kernel-launches", type=int, default=300)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(p, fields, mask, seconds):
+    """The oracle's FP32 restatement of the update, OpenMP over rows."""
+    import oracle.oracle as O
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    n = p.nx * p.ny
+    steps = 4
+    t0 = time.perf_counter()
+    O.run(p, fields, steps, solid=mask, dtype=np.float32, nthreads=cores)
+    dt1 = time.perf_counter() - t0
+    steps = max(4, int(seconds / max(dt1 / 4, 1e-6)))
+    t0 = time.perf_counter()
+    O.run(p, fields, steps, solid=mask, dtype=np.float32, nthreads=cores)
+    el = time.perf_counter() - t0
+    return {"value": n * steps / el / 1e9, "unit": "Gcell-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{p.nx}x{p.ny} same model/mask/IC, {steps} steps, FP32, "
+                      f"{cores} OpenMP threads, {el:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the multi-GPU (domain-decomposed) path is a design "
+              "only in this build; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import yolohtli_amd as Y
+    from yolohtli_amd import abi_types as T
+    from yolohtli_amd import autotune
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    nx = ny = a.nx
+    model = T.YH_MODEL_3V_FK if a.model == "fk" else T.YH_MODEL_2V_REF
+    masked = a.holes > 0
+    p = T.grid_params(nx, ny, model=model, solidSwitch=int(masked))
+    nv = T.nvars(p)
+    mask = Y.make_holes(nx, ny, a.holes, 3.0, 12.0, Y.seed(1, 0, 1)) if masked else None
+    fields = Y.init_random(nx, ny, Y.seed(1, 0, 2), gate=1.0, nvars=nv)
+    n = nx * ny
+    bytes_per_cell = 2 * nv * 4 + (1 if masked else 0)
+
+    stream = torch.cuda.current_stream()
+    pl = Y.Plan(p)
+    if masked:
+        pl.set_mask(mask)
+    pl.set_state(*fields)
+
+    # ---- kernel choice (untimed): the project's autotuner, or a fixed config
+    if a.cfg == "auto":
+        best = autotune.tune_plan(pl, fields, steps=400, log=log)
+        cfg = best["cfg"]
+    else:
+        K, r, w = (int(x) for x in a.cfg.split(","))
+        cfg = (T.YH_KERNEL_TILED, K, r, w)
+    pl.set_kernel(*cfg)
+    pl.set_state(*fields)
+    log(f"kernel cfg variant={cfg[0]} K={cfg[1]} rows={cfg[2]} waves={cfg[3]}")
+
+    def run(nsteps):
+        done = 0
+        while done < nsteps:
+            m = min(a.tip_every, nsteps - done)
+            pl.step(m, stream)
+            done += m
+            if a.tip_every and pl.count % a.tip_every == 0:
+                pl.track_tips(stream)
+
+    run(a.warmup)
+    torch.cuda.synchronize()
+    pl.clear_tips()
+    log(f"warmup {a.warmup} steps done")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    run(a.steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    log(f"timed {a.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.2f} ms)")
+    if pl.check_finite() != 0:
+        log("state is not finite")
+        sys.exit(1)
+    tips = pl.read_tips()
+    value = n * a.steps / (ms / 1e3) / 1e9
+
+    # ---- dominant kernel alone: one launch = K fused steps, events per launch
+    pl.set_option(T.YH_OPT_KEEP_PREV, 0)
+    K = cfg[1] if cfg[0] == T.YH_KERNEL_TILED else 1
+    for _ in range(20):
+        pl.step(K, stream)
+    nl = a.kernel_launches
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * nl)]
+    torch.cuda.synchronize()
+    for q in range(nl):
+        evs[2 * q].record(stream)
+        pl.step(K, stream)
+        evs[2 * q + 1].record(stream)
+    torch.cuda.synchronize()
+    durs = [evs[2 * q].elapsed_time(evs[2 * q + 1]) for q in range(nl)]
+    t_launch_ms = float(np.mean(durs))
+    alg_bytes = n * bytes_per_cell  # per launch, independent of K
+    achieved = alg_bytes / (t_launch_ms / 1e3) / 1e9
+    pl.set_option(T.YH_OPT_KEEP_PREV, 1)
+    log(f"kernel: {t_launch_ms * 1e3:.2f} us per launch of K={K} "
+        f"({n * K / (t_launch_ms / 1e3) / 1e9:.1f} Gcell-steps/s inside one launch)")
+
+    cpu = None
+    if not a.no_cpu:
+        log("CPU baseline ...")
+        cpu = cpu_baseline(p, fields, mask, a.cpu_seconds)
+        log(f"CPU baseline {cpu['value']:.3f} Gcell-steps/s on {cpu['cores']} threads")
+
+    kname = "k_tiled" if cfg[0] == T.YH_KERNEL_TILED else "k_direct_codes"
+    out = {
+        "metric": "Gcell-steps/s + % HBM roofline at 1024^2; batched and adjoint vs plain",
+        "value": round(value, 3),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded splitmix64 holes and random IC)",
+        "config": {
+            "workload": f"plain stepper {nx}x{ny}, {'FK 3V' if nv == 3 else '2V reference'} "
+                        f"model, {a.holes} holes, {a.steps} steps, tips every {a.tip_every}",
+            "nx": nx, "ny": ny, "holes": a.holes, "dt_ms": 0.02, "dx_cm": 0.025,
+            "D_cm2_per_ms": 0.001, "kernel": {"name": kname, "K": cfg[1], "rows": cfg[2],
+                                              "waves": cfg[3]},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "kernel": kname,
+            "launch_us": round(t_launch_ms * 1e3, 3),
+            "alg_bytes_per_launch": alg_bytes,
+            "steps_per_launch": K,
+            "equiv_GBps_per_step": round(bytes_per_cell * n * K / (t_launch_ms / 1e3) / 1e9, 1),
+        },
+        "cpu_baseline": cpu,
+        "tips_logged": int(len(tips)),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * yolohtli.h -- C ABI of the MI355X-native tissue time-stepper.
+ *
+ * Pure C: no HIP or torch types appear in any signature.  Streams are passed
+ * as `void*` (a hipStream_t, or NULL for the null stream).  Every entry point
+ * returns an int error code (YH_OK == 0) and never calls exit().
+ *
+ * Ownership: the caller owns host arrays; a plan owns the device buffers it
+ * allocates.  Functions named *_dev operate on caller-owned device pointers,
+ * exactly as the reference's host wrappers do.  One plan is used by one host
+ * thread at a time; every call is ordered on the stream it is given.
+ *
+ * Memory layout of every field: structure of arrays, float32, row-major with
+ * index i + nx*j (reference globalVariables.cuh:4 I2D, reactionDiffusion.cu:47).
+ * Masks are uint8, 1 = tissue, 0 = hole (main.cu:679 `mesh > 0.5`).
+ */
+#ifndef YOLOHTLI_H
+#define YOLOHTLI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+enum {
+  YH_OK = 0,
+  YH_EINVAL = 1,       /* bad argument (NULL pointer, size, range) */
+  YH_EUNSUPPORTED = 2, /* branch outside the built surface: Dirichlet, RK2/RK4,
+                          lap4, anisotropic cross term */
+  YH_EHIP = 3,         /* the HIP runtime reported an error */
+  YH_ENOMEM = 4,       /* device or host allocation failed */
+  YH_EUNSTABLE = 5,    /* explicit Euler diffusion number rx+ry > 0.5 */
+  YH_ESINGULAR = 6,    /* symmetry-reduction 3x3 solve hit a zero pivot */
+  YH_ENOTIP = 7,       /* no spiral tip found (informational) */
+  YH_EOVERFLOW = 8,    /* tip log saturated; later tips were dropped */
+  YH_ENONFINITE = 9,   /* NaN/Inf found in the state */
+  YH_ESTATE = 10       /* call made in the wrong order */
+};
+
+/* Human-readable text for an error code. */
+const char* yh_strerror(int code);
+
+/* ------------------------------------------------------------ parameters */
+enum { YH_MODEL_2V_REF = 0, YH_MODEL_3V_FK = 1 };
+
+/* Fenton-Karma three-variable model (Fenton & Karma, Chaos 8:20, 1998).
+ * Not present in the reference; defaults are the paper's Beeler-Reuter fit. */
+typedef struct yh_fk_params {
+  double tau_d, tau_r, tau_si, tau_0; /* ms: fast-in, slow-out, slow-in, outward */
+  double tau_vp, tau_v1m, tau_v2m;    /* ms: v gate closing / opening (u<u_v, u>=u_v) */
+  double tau_wp, tau_wm;              /* ms: w gate closing / opening */
+  double u_c, u_v, u_csi, k;          /* thresholds and tanh steepness */
+} yh_fk_params;
+
+/* Solver subset of the reference paramVar (typeDefinition.cuh:35-125), same
+ * field names and meaning.  Derived quantities follow saveFiles.cu:144-170 and
+ * main.cu:148-155: hx = Lx/(nx-1); dt is halved when reduceSym is set. */
+typedef struct yh_params {
+  int nx, ny;
+  double Lx, Ly;                 /* cm */
+  double dt;                     /* ms, before the symmetry-reduction halving */
+  double diff_par, diff_per;     /* cm^2/ms */
+  double degrad;                 /* fibre angle in degrees */
+  double rscale;                 /* gate diffusion scale (2V model) */
+  double tc, alpha, beta, gamma, delta, eps, mu, theta; /* 2V reaction */
+  int model;                     /* YH_MODEL_2V_REF or YH_MODEL_3V_FK */
+  int neumannBC, solidSwitch, gateDiff, reduceSym;
+  int timeIntOrder, lap4, anisotropy; /* only 1, 0, 0 are built */
+  double boundaryVal, Uth;
+  int tipOffsetX, tipOffsetY;
+  yh_fk_params fk;
+  double heaviside_width;        /* 0: sharp STEP(a,b)=(a>=b); >0: 0.5(1+tanh(x/w)) */
+} yh_params;
+
+/* Quantities derived once on the host (saveFiles.cu:144-170, main.cu:148-155). */
+typedef struct yh_derived {
+  double hx, hy, dt, Dxx, Dyy, rx, ry, invdx, invdy;
+} yh_derived;
+
+/* Defaults of the reference parameterSetup (saveFiles.cu:105-231), with the
+ * built branches selected: Euler (timeIntOrder=1), 5-point (lap4=0). */
+void yh_params_default(yh_params* p);
+/* Fenton-Karma 1998 Beeler-Reuter parameter set. */
+void yh_fk_params_default(yh_fk_params* fk);
+/* Validate and derive; returns YH_EUNSUPPORTED for unbuilt branches. */
+int yh_params_derive(const yh_params* p, yh_derived* d);
+
+/* ------------------------------------------------------------- workloads */
+/* splitmix64 sub-stream seed for (base, config, tissue, purpose). */
+uint64_t yh_seed(uint64_t base, uint32_t config, uint32_t tissue, uint32_t purpose);
+/* Cross-field initial condition (main.cu:577-621): u=1 for i<u_cols, gate
+ * variables = refr for j>=ny/2 and = rest otherwise.  w may be NULL. */
+int yh_init_cross_field(int nx, int ny, int u_cols, float rest, float refr,
+                        float* u, float* v, float* w);
+/* u ~ U[0,1) i.i.d. from splitmix64(seed); v (and w) = gate. */
+int yh_init_random(int nx, int ny, uint64_t seed, float gate,
+                   float* u, float* v, float* w);
+/* Seeded circular holes (replaces the missing common/holes*.dat files read at
+ * main.cu:634-684): centres U[0,nx)xU[0,ny), radii U[rmin,rmax] cells; cell is
+ * a hole when (i-cx)^2+(j-cy)^2 <= r^2.  mask: 1 = tissue, 0 = hole. */
+int yh_make_holes(int nx, int ny, int nholes, double rmin, double rmax,
+                  uint64_t seed, uint8_t* mask);
+
+/* ---------------------------------------------- single-step device wrapper */
+/* One explicit-Euler step on caller-owned device arrays.  Replaces
+ * reactionDiffusion_wrapper (reactionDiffusion.cu:568-577): in[] are read,
+ * out[] written (never aliasing in[]), veltan[] (may be NULL) receives
+ * rhs/dt (reactionDiffusion.cu:529-530,551-552).  Arrays: u,v for the 2V
+ * model; u,v,w for the FK model.  solid may be NULL unless solidSwitch. */
+int yh_rd_step_dev(const yh_params* p, float* const out[3], const float* const in[3],
+                   float* const veltan[3], const uint8_t* solid, void* stream);
+
+/* Spiral-tip record: the reference vec5dyn (typeDefinition.cuh:19-20). */
+typedef struct yh_tip { float x, y, vx, vy, t; } yh_tip;
+
+/* Spiral-tip detection between two u fields on the device.  Replaces
+ * tip_wrapper with tipAlgorithm 1 (tipTracker.cu:20-241,569-611): g_past and
+ * g_present keep the reference's roles.  Found tips are appended to
+ * d_tips/d_keys at positions [*d_count, ...) (d_count is NOT reset), keys are
+ * 2*(i+nx*j)+root for deterministic ordering; entries past cap are dropped
+ * while *d_count still counts them. */
+int yh_tip_detect_dev(const yh_params* p, const float* g_past, const float* g_present,
+                      double time, yh_tip* d_tips, int* d_keys, int* d_count, int cap,
+                      void* stream);
+
+/* ------------------------------------------------------------------ plans */
+typedef struct yh_plan yh_plan;
+
+/* Kernel choice for the plain stepper. */
+enum { YH_KERNEL_AUTO = 0, YH_KERNEL_DIRECT = 1, YH_KERNEL_TILED = 2 };
+typedef struct yh_kernel_cfg {
+  int variant;   /* YH_KERNEL_* */
+  int substeps;  /* k fused time steps per launch (TILED), 1..16 */
+  int rows;      /* rows held in registers per thread (TILED): 4, 8 or 16 */
+  int waves;     /* wavefronts per workgroup (TILED): 4 or 8 */
+} yh_kernel_cfg;
+
+/* Creates a plan: checks parameters, allocates ping-pong state on the current
+ * device.  Mirrors main.cu:213-402 (allocation + constant upload). */
+int yh_plan_create(const yh_params* p, yh_plan** out);
+void yh_plan_destroy(yh_plan* plan);
+int yh_plan_params(const yh_plan* plan, yh_params* p, yh_derived* d);
+/* Mask (1 = tissue).  Required when solidSwitch; stencil codes are built here. */
+int yh_set_mask(yh_plan* plan, const uint8_t* solid);
+/* Host state in/out; w is ignored (may be NULL) for the 2V model. */
+int yh_set_state(yh_plan* plan, const float* u, const float* v, const float* w);
+int yh_get_state(yh_plan* plan, float* u, float* v, float* w);
+/* Device pointers of the current state (valid until the next step call). */
+int yh_state_dev(yh_plan* plan, float** u, float** v, float** w);
+int yh_set_kernel(yh_plan* plan, const yh_kernel_cfg* cfg);
+int yh_get_kernel(const yh_plan* plan, yh_kernel_cfg* cfg);
+/* Advance nsteps explicit-Euler steps (the display() plain loop,
+ * main.cu:876-885).  After return the plan also holds the state one step
+ * earlier, so yh_track_tips sees (u^{n-1}, u^n) as main.cu:961-964 does. */
+int yh_step(yh_plan* plan, int nsteps, void* stream);
+/* Steps taken since the last yh_set_state (param.count, main.cu:884). */
+int64_t yh_step_count(const yh_plan* plan);
+/* Detect tips between the previous and the current state and append them to
+ * the plan's device tip log, stamped t = dt*count (main.cu:961-964). */
+int yh_track_tips(yh_plan* plan, void* stream);
+/* Synchronise and copy the tip log, ordered by (t, cell, root).  *n receives
+ * the number of tips logged (may exceed cap: YH_EOVERFLOW). */
+int yh_read_tips(yh_plan* plan, yh_tip* out, int cap, int* n);
+int yh_clear_tips(yh_plan* plan);
+/* Device flag check: YH_ENONFINITE when any state value is NaN/Inf. */
+int yh_check_finite(yh_plan* plan, void* stream);
+
+/* Plan options.  YH_OPT_KEEP_PREV (default 1): end every yh_step with a single
+ * step so the previous state is kept for yh_track_tips; 0 lets yh_step use
+ * only k-substep launches (no tip tracking until the next yh_step with 1). */
+enum { YH_OPT_KEEP_PREV = 1 };
+int yh_set_option(yh_plan* plan, int option, int64_t value);
+
+/* Autotuner (no reference counterpart; the reference launches fixed 16x16
+ * blocks, globalVariables.cuh:15-16): times the direct kernel and the tiled
+ * kernel over tile shapes x substep depths on the plan's own state (restored
+ * afterwards), installs the fastest, and caches it per (nx, ny, model,
+ * masked) for the process.  best / best_ms (ms per step) may be NULL. */
+int yh_autotune(yh_plan* plan, int steps, void* stream, yh_kernel_cfg* best, float* best_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YOLOHTLI_H */

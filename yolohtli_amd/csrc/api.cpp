@@ -1,0 +1,284 @@
+// api.cpp -- C ABI: parameters, workload generators, single-step device
+// wrappers and the plan object of the plain stepper.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "plan.h"
+#include "yh_internal.h"
+
+namespace yh {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+
+int hip_fail(hipError_t e, const char* what) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  set_last_error(buf);
+  if (e == hipErrorOutOfMemory) return YH_ENOMEM;
+  return YH_EHIP;
+}
+
+int nvars_of(int model) { return model == YH_MODEL_3V_FK ? 3 : 2; }
+
+void make_kparams(const yh_params& p, const yh_derived& d, KParams* k) {
+  k->nx = p.nx;
+  k->ny = p.ny;
+  k->rx = (float)d.rx;
+  k->ry = (float)d.ry;
+  k->dt = (float)d.dt;
+  k->tc = (float)p.tc;
+  k->tcdt = (float)(p.tc * d.dt);
+  k->rs = p.gateDiff ? (float)p.rscale : 0.0f;
+  k->mu = (float)p.mu;
+  k->alpha = (float)p.alpha;
+  k->beta = (float)p.beta;
+  k->gamma = (float)p.gamma;
+  k->delta = (float)p.delta;
+  k->eps = (float)p.eps;
+  k->theta = (float)p.theta;
+  const yh_fk_params& f = p.fk;
+  k->i_td = (float)(1.0 / f.tau_d);
+  k->i_tr = (float)(1.0 / f.tau_r);
+  k->i_2tsi = (float)(1.0 / (2.0 * f.tau_si));
+  k->i_t0 = (float)(1.0 / f.tau_0);
+  k->i_tvp = (float)(1.0 / f.tau_vp);
+  k->i_tv1m = (float)(1.0 / f.tau_v1m);
+  k->i_tv2m = (float)(1.0 / f.tau_v2m);
+  k->i_twp = (float)(1.0 / f.tau_wp);
+  k->i_twm = (float)(1.0 / f.tau_wm);
+  k->u_c = (float)f.u_c;
+  k->u_v = (float)f.u_v;
+  k->u_csi = (float)f.u_csi;
+  k->kk = (float)f.k;
+  k->hw = (float)p.heaviside_width;
+}
+
+// splitmix64 (Steele, Lea & Flood 2014).
+struct SplitMix {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+}  // namespace yh
+
+using namespace yh;
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* yh_strerror(int code) {
+  switch (code) {
+    case YH_OK: return "ok";
+    case YH_EINVAL: return "invalid argument";
+    case YH_EUNSUPPORTED: return "branch not built (Dirichlet, RK2/RK4, lap4, anisotropy)";
+    case YH_EHIP: return g_last_error.empty() ? "HIP runtime error" : g_last_error.c_str();
+    case YH_ENOMEM: return "out of memory";
+    case YH_EUNSTABLE: return "explicit Euler diffusion number rx+ry exceeds 0.5";
+    case YH_ESINGULAR: return "symmetry-reduction matrix is singular";
+    case YH_ENOTIP: return "no spiral tip found";
+    case YH_EOVERFLOW: return "tip log overflow";
+    case YH_ENONFINITE: return "non-finite value in the state";
+    case YH_ESTATE: return "call out of order";
+    default: return "unknown error";
+  }
+}
+
+void yh_fk_params_default(yh_fk_params* f) {
+  if (!f) return;
+  // Fenton & Karma (1998), Table 1, Beeler-Reuter fit.
+  f->tau_d = 0.25;  // C_m/g_fi with g_fi = 4
+  f->tau_r = 33.33;
+  f->tau_si = 29.0;
+  f->tau_0 = 12.5;
+  f->tau_vp = 3.33;
+  f->tau_v1m = 1250.0;
+  f->tau_v2m = 19.6;
+  f->tau_wp = 870.0;
+  f->tau_wm = 41.0;
+  f->u_c = 0.13;
+  f->u_v = 0.04;
+  f->u_csi = 0.85;
+  f->k = 10.0;
+}
+
+void yh_params_default(yh_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  // parameterSetup, saveFiles.cu:105-231 (solver subset).
+  p->nx = 512;
+  p->ny = 512;
+  p->Lx = 12.0;
+  p->Ly = 12.0;
+  p->dt = 0.02;
+  p->diff_par = 0.001;
+  p->diff_per = 0.001;
+  p->degrad = 0.0;
+  p->rscale = 0.01;
+  p->tc = 1.0;
+  p->alpha = 0.2;
+  p->beta = 1.1;
+  p->gamma = 0.0;
+  p->delta = 1.0;
+  p->eps = 0.005;
+  p->mu = 1.0;
+  p->theta = 0.0;
+  p->model = YH_MODEL_2V_REF;
+  p->neumannBC = 1;
+  p->solidSwitch = 0;
+  p->gateDiff = 1;
+  p->reduceSym = 0;
+  // The reference defaults are RK4 + lap4 (saveFiles.cu:131-132); the built
+  // branch is Euler + 5-point.
+  p->timeIntOrder = 1;
+  p->lap4 = 0;
+  p->anisotropy = 0;
+  p->boundaryVal = 0.0;
+  p->Uth = 0.7;
+  p->tipOffsetX = 160;
+  p->tipOffsetY = 160;
+  yh_fk_params_default(&p->fk);
+  p->heaviside_width = 0.0;
+}
+
+int yh_params_derive(const yh_params* p, yh_derived* d) {
+  if (!p || !d) return YH_EINVAL;
+  if (p->nx < 4 || p->ny < 4) return YH_EINVAL;
+  if ((int64_t)p->nx * p->ny > (int64_t)1 << 31) return YH_EINVAL;
+  if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->dt > 0)) return YH_EINVAL;
+  if (p->model != YH_MODEL_2V_REF && p->model != YH_MODEL_3V_FK) return YH_EINVAL;
+  if (!p->neumannBC || p->timeIntOrder != 1 || p->lap4 != 0 || p->anisotropy) return YH_EUNSUPPORTED;
+  if (p->model == YH_MODEL_3V_FK) {
+    const yh_fk_params& f = p->fk;
+    if (!(f.tau_d > 0 && f.tau_r > 0 && f.tau_si > 0 && f.tau_0 > 0 && f.tau_vp > 0 &&
+          f.tau_v1m > 0 && f.tau_v2m > 0 && f.tau_wp > 0 && f.tau_wm > 0))
+      return YH_EINVAL;
+  }
+  const double pi = 3.14159265359;  // the reference's pi (typeDefinition.cuh:152)
+  d->hx = p->Lx / (p->nx - 1.0);
+  d->hy = p->Ly / (p->ny - 1.0);
+  const double th = p->degrad * pi / 180.0;
+  d->Dxx = p->diff_par * std::cos(th) * std::cos(th) + p->diff_per * std::sin(th) * std::sin(th);
+  d->Dyy = p->diff_par * std::sin(th) * std::sin(th) + p->diff_per * std::cos(th) * std::cos(th);
+  d->dt = p->reduceSym ? 0.5 * p->dt : p->dt;
+  d->rx = d->dt * d->Dxx / (d->hx * d->hx);
+  d->ry = d->dt * d->Dyy / (d->hy * d->hy);
+  d->invdx = 0.5 / d->hx;
+  d->invdy = 0.5 / d->hy;
+  if (d->rx + d->ry > 0.5) return YH_EUNSTABLE;
+  return YH_OK;
+}
+
+uint64_t yh_seed(uint64_t base, uint32_t config, uint32_t tissue, uint32_t purpose) {
+  SplitMix m{base};
+  uint64_t h = m.next();
+  const uint32_t parts[3] = {config, tissue, purpose};
+  for (uint32_t v : parts) {
+    SplitMix q{h ^ ((uint64_t)v * 0xD1B54A32D192ED03ull)};
+    h = q.next();
+  }
+  return h;
+}
+
+int yh_init_cross_field(int nx, int ny, int u_cols, float rest, float refr, float* u, float* v,
+                        float* w) {
+  if (nx <= 0 || ny <= 0 || !u || !v) return YH_EINVAL;
+  // initGates, main.cu:593-618: u = 1 for i < nx/8 (here u_cols), v = 1 for j >= ny/2.
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const size_t c = (size_t)i + (size_t)nx * j;
+      u[c] = i < u_cols ? 1.0f : 0.0f;
+      const float g = j >= ny / 2 ? refr : rest;
+      v[c] = g;
+      if (w) w[c] = g;
+    }
+  return YH_OK;
+}
+
+int yh_init_random(int nx, int ny, uint64_t seed, float gate, float* u, float* v, float* w) {
+  if (nx <= 0 || ny <= 0 || !u || !v) return YH_EINVAL;
+  SplitMix m{seed};
+  const size_t n = (size_t)nx * ny;
+  for (size_t c = 0; c < n; ++c) {
+    u[c] = (float)m.uniform();
+    v[c] = gate;
+    if (w) w[c] = gate;
+  }
+  return YH_OK;
+}
+
+int yh_make_holes(int nx, int ny, int nholes, double rmin, double rmax, uint64_t seed,
+                  uint8_t* mask) {
+  if (nx <= 0 || ny <= 0 || nholes < 0 || !mask || rmin < 0 || rmax < rmin) return YH_EINVAL;
+  std::memset(mask, 1, (size_t)nx * ny);
+  SplitMix m{seed};
+  for (int h = 0; h < nholes; ++h) {
+    const double cx = m.uniform() * nx;
+    const double cy = m.uniform() * ny;
+    const double r = rmin + m.uniform() * (rmax - rmin);
+    const int i0 = std::max(0, (int)std::floor(cx - r)), i1 = std::min(nx - 1, (int)std::ceil(cx + r));
+    const int j0 = std::max(0, (int)std::floor(cy - r)), j1 = std::min(ny - 1, (int)std::ceil(cy + r));
+    for (int j = j0; j <= j1; ++j)
+      for (int i = i0; i <= i1; ++i) {
+        const double dx = i - cx, dy = j - cy;
+        if (dx * dx + dy * dy <= r * r) mask[(size_t)i + (size_t)nx * j] = 0;
+      }
+  }
+  return YH_OK;
+}
+
+int yh_rd_step_dev(const yh_params* p, float* const out[3], const float* const in[3],
+                   float* const veltan[3], const uint8_t* solid, void* stream) {
+  yh_derived d;
+  int rc = yh_params_derive(p, &d);
+  if (rc) return rc;
+  const int nv = nvars_of(p->model);
+  if (!out || !in) return YH_EINVAL;
+  for (int a = 0; a < nv; ++a) {
+    if (!out[a] || !in[a]) return YH_EINVAL;
+    if (veltan && !veltan[a]) return YH_EINVAL;
+    for (int b = 0; b < nv; ++b)
+      if ((const float*)out[a] == in[b]) return YH_EINVAL;
+  }
+  if (p->solidSwitch && !solid) return YH_EINVAL;
+  KParams k;
+  make_kparams(*p, d, &k);
+  CFields ci{{in[0], in[1], nv > 2 ? in[2] : nullptr}};
+  Fields co{{out[0], out[1], nv > 2 ? out[2] : nullptr}};
+  Fields cv{{nullptr, nullptr, nullptr}};
+  if (veltan) cv = Fields{{veltan[0], veltan[1], nv > 2 ? veltan[2] : nullptr}};
+  YH_HIP(launch_direct(p->model, k, ci, co, cv, p->solidSwitch ? solid : nullptr,
+                       (hipStream_t)stream));
+  return YH_OK;
+}
+
+int yh_tip_detect_dev(const yh_params* p, const float* g_past, const float* g_present,
+                      double time, yh_tip* d_tips, int* d_keys, int* d_count, int cap,
+                      void* stream) {
+  yh_derived d;
+  int rc = yh_params_derive(p, &d);
+  if (rc) return rc;
+  if (!g_past || !g_present || !d_tips || !d_keys || !d_count || cap < 0) return YH_EINVAL;
+  KParams k;
+  make_kparams(*p, d, &k);
+  YH_HIP(launch_tips(k, p->solidSwitch, p->tipOffsetX, p->tipOffsetY, p->Uth, g_past, g_present,
+                     time, d_tips, d_keys, d_count, cap, (hipStream_t)stream));
+  return YH_OK;
+}
+
+}  // extern "C"

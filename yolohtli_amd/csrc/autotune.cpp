@@ -1,0 +1,120 @@
+// autotune.cpp -- picks the plain-stepper kernel (tile shape x substep depth)
+// by timing candidates on the plan's own state with HIP events.  Results are
+// cached per (nx, ny, model, masked) for the life of the process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "plan.h"
+
+namespace {
+
+using Key = std::tuple<int, int, int, int>;
+std::mutex g_mu;
+std::map<Key, std::pair<yh_kernel_cfg, float>> g_cache;
+
+std::vector<yh_kernel_cfg> candidates(const yh_plan* pl) {
+  std::vector<yh_kernel_cfg> c;
+  c.push_back({YH_KERNEL_DIRECT, 1, 0, 0});
+  const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
+  const int Ks[] = {1, 2, 3, 4, 6, 8, 12};
+  const int shapes[][2] = {{4, 8}, {8, 4}, {8, 8}, {16, 4}, {16, 8}, {4, 4}};
+  for (int K : Ks) {
+    if (K > kmax) continue;
+    for (auto& s : shapes) {
+      const int rows = s[0], waves = s[1];
+      if (2 * K >= 64 || 2 * K >= rows * waves) continue;
+      // FK keeps three fields per row in registers: 16 rows spills occupancy.
+      if (pl->p.model == YH_MODEL_3V_FK && rows == 16) continue;
+      c.push_back({YH_KERNEL_TILED, K, rows, waves});
+    }
+  }
+  return c;
+}
+
+}  // namespace
+
+using namespace yh;
+
+extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* best,
+                           float* best_ms) {
+  if (!pl || steps < 1) return YH_EINVAL;
+  if (pl->p.solidSwitch && !pl->mask_set) return YH_ESTATE;
+  hipStream_t s = (hipStream_t)stream;
+  const Key key{pl->p.nx, pl->p.ny, pl->p.model, pl->p.solidSwitch};
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) {
+      pl->cfg = it->second.first;
+      if (best) *best = it->second.first;
+      if (best_ms) *best_ms = it->second.second;
+      return YH_OK;
+    }
+  }
+  // Save the state, run the candidates, restore.
+  const size_t bytes = pl->n * sizeof(float);
+  float* save[3] = {nullptr, nullptr, nullptr};
+  for (int a = 0; a < pl->nv; ++a) {
+    YH_HIP(hipMalloc(&save[a], bytes));
+    YH_HIP(hipMemcpyAsync(save[a], pl->buf[pl->cur][a], bytes, hipMemcpyDeviceToDevice, s));
+  }
+  const int64_t count0 = pl->count;
+  const bool keep0 = pl->keep_prev, prev0 = pl->have_prev;
+  const yh_kernel_cfg cfg0 = pl->cfg;
+  pl->keep_prev = false;
+  hipEvent_t e0, e1;
+  YH_HIP(hipEventCreate(&e0));
+  YH_HIP(hipEventCreate(&e1));
+  yh_kernel_cfg bestc = cfg0;
+  float bestt = 1e30f;
+  int rc = YH_OK;
+  for (const yh_kernel_cfg& c : candidates(pl)) {
+    pl->cfg = c;
+    const int K = c.variant == YH_KERNEL_TILED ? c.substeps : 1;
+    const int n = std::max(K, (steps / K) * K);
+    rc = plan_advance(pl, K, s);  // warm-up launch (code object load)
+    if (rc) break;
+    if (hipEventRecord(e0, s) != hipSuccess) { rc = YH_EHIP; break; }
+    rc = plan_advance(pl, n, s);
+    if (rc) break;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      rc = YH_EHIP;
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const float per_step = ms / n;
+    if (per_step < bestt) {
+      bestt = per_step;
+      bestc = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int a = 0; a < pl->nv; ++a) {
+    (void)hipMemcpyAsync(pl->buf[pl->cur][a], save[a], bytes, hipMemcpyDeviceToDevice, s);
+  }
+  (void)hipStreamSynchronize(s);
+  for (int a = 0; a < pl->nv; ++a) (void)hipFree(save[a]);
+  pl->count = count0;
+  pl->keep_prev = keep0;
+  (void)prev0;
+  pl->have_prev = false;  // buf[cur^1] holds a candidate's output, not the previous step
+  if (rc) {
+    pl->cfg = cfg0;
+    return rc;
+  }
+  pl->cfg = bestc;
+  {
+    std::lock_guard<std::mutex> g(g_mu);
+    g_cache[key] = {bestc, bestt};
+  }
+  if (best) *best = bestc;
+  if (best_ms) *best_ms = bestt;
+  return YH_OK;
+}

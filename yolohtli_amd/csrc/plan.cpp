@@ -1,0 +1,268 @@
+// plan.cpp -- yh_plan: owns the ping-pong state, mask codes and tip log of
+// one tissue; drives the plain stepper (the display() loop, main.cu:876-885).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "plan.h"
+
+namespace yh {
+
+yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
+  yh_kernel_cfg c = pl->cfg;
+  if (c.variant == YH_KERNEL_AUTO) {
+    c.variant = YH_KERNEL_TILED;
+    c.substeps = 4;
+    c.rows = 8;
+    c.waves = 8;
+  }
+  if (c.variant == YH_KERNEL_TILED) {
+    const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
+    c.substeps = std::max(1, std::min(c.substeps, kmax));
+  }
+  return c;
+}
+
+int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
+  if (nsteps < 0) return YH_EINVAL;
+  if (nsteps == 0) return YH_OK;
+  if (pl->p.solidSwitch && !pl->mask_set) return YH_ESTATE;
+  const yh_kernel_cfg c = resolve_cfg(pl);
+  const bool masked = pl->p.solidSwitch != 0;
+  int remaining = nsteps;
+  auto fields = [&](int b, CFields* ci, Fields* co) {
+    *ci = CFields{{pl->buf[b][0], pl->buf[b][1], pl->buf[b][2]}};
+    *co = Fields{{pl->buf[b ^ 1][0], pl->buf[b ^ 1][1], pl->buf[b ^ 1][2]}};
+  };
+  const int tail = pl->keep_prev ? 1 : 0;
+  if (c.variant == YH_KERNEL_TILED) {
+    while (remaining > tail) {
+      const int K = std::min(c.substeps, remaining - tail);
+      CFields ci;
+      Fields co;
+      fields(pl->cur, &ci, &co);
+      YH_HIP(launch_tiled(pl->p.model, masked, c.rows, c.waves, K, pl->k, ci, co,
+                          masked ? pl->codes_d : nullptr, s));
+      pl->cur ^= 1;
+      remaining -= K;
+    }
+  }
+  while (remaining > 0) {
+    CFields ci;
+    Fields co;
+    fields(pl->cur, &ci, &co);
+    YH_HIP(launch_direct_codes(pl->p.model, pl->k, ci, co, masked ? pl->codes_d : nullptr, s));
+    pl->cur ^= 1;
+    remaining -= 1;
+  }
+  pl->have_prev = c.variant != YH_KERNEL_TILED || pl->keep_prev || c.substeps == 1;
+  pl->count += nsteps;
+  return YH_OK;
+}
+
+static void plan_free(yh_plan* pl) {
+  if (!pl) return;
+  for (int b = 0; b < 2; ++b)
+    for (int a = 0; a < 3; ++a)
+      if (pl->buf[b][a]) (void)hipFree(pl->buf[b][a]);
+  if (pl->solid_d) (void)hipFree(pl->solid_d);
+  if (pl->codes_d) (void)hipFree(pl->codes_d);
+  if (pl->tips_d) (void)hipFree(pl->tips_d);
+  if (pl->keys_d) (void)hipFree(pl->keys_d);
+  if (pl->tipcount_d) (void)hipFree(pl->tipcount_d);
+  if (pl->flag_d) (void)hipFree(pl->flag_d);
+  if (pl->sr) sr_free(pl->sr);
+  delete pl;
+}
+
+}  // namespace yh
+
+using namespace yh;
+
+extern "C" {
+
+int yh_plan_create(const yh_params* p, yh_plan** out) {
+  if (!p || !out) return YH_EINVAL;
+  *out = nullptr;
+  yh_derived d;
+  int rc = yh_params_derive(p, &d);
+  if (rc) return rc;
+  yh_plan* pl = new (std::nothrow) yh_plan();
+  if (!pl) return YH_ENOMEM;
+  pl->p = *p;
+  pl->d = d;
+  make_kparams(*p, d, &pl->k);
+  pl->nv = nvars_of(p->model);
+  pl->n = (size_t)p->nx * (size_t)p->ny;
+  pl->tipcap = 500000;  // TIPVECSIZE, globalVariables.cuh:12
+  hipError_t e = hipGetDevice(&pl->device);
+  const size_t bytes = pl->n * sizeof(float);
+  for (int b = 0; b < 2 && e == hipSuccess; ++b)
+    for (int a = 0; a < pl->nv && e == hipSuccess; ++a) e = hipMalloc(&pl->buf[b][a], bytes);
+  if (e == hipSuccess) e = hipMalloc(&pl->tips_d, sizeof(yh_tip) * pl->tipcap);
+  if (e == hipSuccess) e = hipMalloc(&pl->keys_d, sizeof(int) * pl->tipcap);
+  if (e == hipSuccess) e = hipMalloc(&pl->tipcount_d, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&pl->flag_d, sizeof(int));
+  if (e == hipSuccess) e = hipMemset(pl->tipcount_d, 0, sizeof(int));
+  for (int b = 0; b < 2 && e == hipSuccess; ++b)
+    for (int a = 0; a < pl->nv && e == hipSuccess; ++a) e = hipMemset(pl->buf[b][a], 0, bytes);
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "yh_plan_create");
+    plan_free(pl);
+    return rc;
+  }
+  *out = pl;
+  return YH_OK;
+}
+
+void yh_plan_destroy(yh_plan* plan) {
+  if (plan) (void)hipDeviceSynchronize();
+  plan_free(plan);
+}
+
+int yh_plan_params(const yh_plan* pl, yh_params* p, yh_derived* d) {
+  if (!pl) return YH_EINVAL;
+  if (p) *p = pl->p;
+  if (d) *d = pl->d;
+  return YH_OK;
+}
+
+int yh_set_mask(yh_plan* pl, const uint8_t* solid) {
+  if (!pl || !solid) return YH_EINVAL;
+  if (!pl->solid_d) {
+    YH_HIP(hipMalloc(&pl->solid_d, pl->n));
+    YH_HIP(hipMalloc(&pl->codes_d, pl->n));
+  }
+  std::vector<uint8_t> m(pl->n);
+  for (size_t c = 0; c < pl->n; ++c) m[c] = solid[c] ? 1 : 0;
+  YH_HIP(hipMemcpy(pl->solid_d, m.data(), pl->n, hipMemcpyHostToDevice));
+  YH_HIP(launch_codes(pl->solid_d, pl->codes_d, pl->p.nx, pl->p.ny, nullptr));
+  YH_HIP(hipDeviceSynchronize());
+  pl->mask_set = true;
+  return YH_OK;
+}
+
+int yh_set_state(yh_plan* pl, const float* u, const float* v, const float* w) {
+  if (!pl || !u || !v || (pl->nv == 3 && !w)) return YH_EINVAL;
+  const float* src[3] = {u, v, w};
+  const size_t bytes = pl->n * sizeof(float);
+  YH_HIP(hipDeviceSynchronize());
+  for (int a = 0; a < pl->nv; ++a)
+    YH_HIP(hipMemcpy(pl->buf[pl->cur][a], src[a], bytes, hipMemcpyHostToDevice));
+  pl->have_prev = false;
+  pl->count = 0;
+  return YH_OK;
+}
+
+int yh_get_state(yh_plan* pl, float* u, float* v, float* w) {
+  if (!pl || !u || !v || (pl->nv == 3 && !w)) return YH_EINVAL;
+  float* dst[3] = {u, v, w};
+  const size_t bytes = pl->n * sizeof(float);
+  YH_HIP(hipDeviceSynchronize());
+  for (int a = 0; a < pl->nv; ++a)
+    YH_HIP(hipMemcpy(dst[a], pl->buf[pl->cur][a], bytes, hipMemcpyDeviceToHost));
+  return YH_OK;
+}
+
+int yh_state_dev(yh_plan* pl, float** u, float** v, float** w) {
+  if (!pl) return YH_EINVAL;
+  if (u) *u = pl->buf[pl->cur][0];
+  if (v) *v = pl->buf[pl->cur][1];
+  if (w) *w = pl->nv == 3 ? pl->buf[pl->cur][2] : nullptr;
+  return YH_OK;
+}
+
+int yh_set_kernel(yh_plan* pl, const yh_kernel_cfg* cfg) {
+  if (!pl || !cfg) return YH_EINVAL;
+  if (cfg->variant < YH_KERNEL_AUTO || cfg->variant > YH_KERNEL_TILED) return YH_EINVAL;
+  if (cfg->variant == YH_KERNEL_TILED) {
+    if (!tiled_supported(cfg->rows, cfg->waves)) return YH_EINVAL;
+    if (cfg->substeps < 1 || cfg->substeps > 16) return YH_EINVAL;
+    if (2 * cfg->substeps >= 64 || 2 * cfg->substeps >= cfg->rows * cfg->waves) return YH_EINVAL;
+  }
+  pl->cfg = *cfg;
+  return YH_OK;
+}
+
+int yh_get_kernel(const yh_plan* pl, yh_kernel_cfg* cfg) {
+  if (!pl || !cfg) return YH_EINVAL;
+  *cfg = resolve_cfg(pl);
+  return YH_OK;
+}
+
+int yh_step(yh_plan* pl, int nsteps, void* stream) {
+  if (!pl) return YH_EINVAL;
+  return plan_advance(pl, nsteps, (hipStream_t)stream);
+}
+
+int64_t yh_step_count(const yh_plan* pl) { return pl ? pl->count : -1; }
+
+int yh_track_tips(yh_plan* pl, void* stream) {
+  if (!pl) return YH_EINVAL;
+  if (!pl->have_prev) return YH_ESTATE;
+  // main.cu:961-964 after swapSoA: g_past = previous state, g_present = newest.
+  const double t = pl->d.dt * (double)pl->count;
+  YH_HIP(launch_tips(pl->k, pl->p.solidSwitch, pl->p.tipOffsetX, pl->p.tipOffsetY, pl->p.Uth,
+                     pl->buf[pl->cur ^ 1][0], pl->buf[pl->cur][0], t, pl->tips_d, pl->keys_d,
+                     pl->tipcount_d, pl->tipcap, (hipStream_t)stream));
+  return YH_OK;
+}
+
+int yh_read_tips(yh_plan* pl, yh_tip* out, int cap, int* n) {
+  if (!pl || !n || cap < 0 || (cap > 0 && !out)) return YH_EINVAL;
+  YH_HIP(hipDeviceSynchronize());
+  int cnt = 0;
+  YH_HIP(hipMemcpy(&cnt, pl->tipcount_d, sizeof(int), hipMemcpyDeviceToHost));
+  const int have = std::min(cnt, pl->tipcap);
+  std::vector<yh_tip> t(have);
+  std::vector<int> k(have);
+  if (have > 0) {
+    YH_HIP(hipMemcpy(t.data(), pl->tips_d, sizeof(yh_tip) * have, hipMemcpyDeviceToHost));
+    YH_HIP(hipMemcpy(k.data(), pl->keys_d, sizeof(int) * have, hipMemcpyDeviceToHost));
+  }
+  std::vector<int> order(have);
+  for (int q = 0; q < have; ++q) order[q] = q;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    if (t[a].t != t[b].t) return t[a].t < t[b].t;
+    return k[a] < k[b];
+  });
+  const int m = std::min(have, cap);
+  for (int q = 0; q < m; ++q) out[q] = t[order[q]];
+  *n = cnt;
+  return (cnt > pl->tipcap || cnt > cap) ? YH_EOVERFLOW : YH_OK;
+}
+
+int yh_clear_tips(yh_plan* pl) {
+  if (!pl) return YH_EINVAL;
+  YH_HIP(hipDeviceSynchronize());
+  YH_HIP(hipMemset(pl->tipcount_d, 0, sizeof(int)));
+  return YH_OK;
+}
+
+int yh_set_option(yh_plan* pl, int option, int64_t value) {
+  if (!pl) return YH_EINVAL;
+  switch (option) {
+    case YH_OPT_KEEP_PREV:
+      pl->keep_prev = value != 0;
+      return YH_OK;
+    default:
+      return YH_EINVAL;
+  }
+}
+
+int yh_check_finite(yh_plan* pl, void* stream) {
+  if (!pl) return YH_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  YH_HIP(hipMemsetAsync(pl->flag_d, 0, sizeof(int), s));
+  const float* f[3] = {pl->buf[pl->cur][0], pl->buf[pl->cur][1], pl->buf[pl->cur][2]};
+  YH_HIP(launch_finite(f, pl->nv, pl->n, pl->flag_d, s));
+  int flag = 0;
+  YH_HIP(hipMemcpyAsync(&flag, pl->flag_d, sizeof(int), hipMemcpyDeviceToHost, s));
+  YH_HIP(hipStreamSynchronize(s));
+  return flag ? YH_ENONFINITE : YH_OK;
+}
+
+}  // extern "C"

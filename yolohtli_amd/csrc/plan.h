@@ -1,0 +1,41 @@
+// plan.h -- the plan object behind yh_plan (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "yh_internal.h"
+
+struct yh_sr;  // symmetry-reduction session state (sr.hip)
+
+struct yh_plan {
+  yh_params p;
+  yh_derived d;
+  yh::KParams k;
+  int nv = 2;
+  size_t n = 0;
+  int device = 0;
+  float* buf[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  int cur = 0;             // buf[cur] holds the current state
+  bool have_prev = false;  // buf[cur^1] holds the state one step earlier
+  uint8_t* solid_d = nullptr;
+  uint8_t* codes_d = nullptr;
+  bool mask_set = false;
+  yh_tip* tips_d = nullptr;
+  int* keys_d = nullptr;
+  int* tipcount_d = nullptr;
+  int tipcap = 0;
+  int* flag_d = nullptr;
+  yh_kernel_cfg cfg{YH_KERNEL_AUTO, 4, 8, 8};
+  int64_t count = 0;
+  bool keep_prev = true;
+  yh_sr* sr = nullptr;
+};
+
+namespace yh {
+// Resolve YH_KERNEL_AUTO to a concrete configuration for this plan.
+yh_kernel_cfg resolve_cfg(const yh_plan* pl);
+// Advance `nsteps` on the plan's buffers; the last step is a single step so
+// that buf[cur^1] is the previous state.
+int plan_advance(yh_plan* pl, int nsteps, hipStream_t s);
+void sr_free(yh_sr* sr);
+}  // namespace yh

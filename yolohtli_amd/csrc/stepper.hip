@@ -1,0 +1,419 @@
+// stepper.hip -- plain tissue time-stepper kernels for gfx950 (CDNA4).
+//
+// The update is the explicit-Euler, 5-point, no-flux step of
+// reactionDiffusion_kernel (reactionDiffusion.cu:107-199,498-561) with the
+// ionic model as a compile-time functor (Model<M2V>: the reference's 2V
+// reaction; Model<MFK>: Fenton-Karma 3V).
+//
+// Kernels
+//   k_codes        mask -> per-cell 2-bit stencil coefficients (once per mask)
+//   k_direct       one step, one thread per cell, reads the raw mask; writes
+//                  velTan when asked (the reactionDiffusion_wrapper surface)
+//   k_direct_codes one step from stencil codes (the k=1 plan path)
+//   k_tiled        K fused steps per launch (temporal blocking): a workgroup
+//                  owns a 64 x (NW*R) region, each lane one column, each wave
+//                  R consecutive rows kept in registers.  West/east neighbours
+//                  come from ds_bpermute (no LDS memory), north/south from the
+//                  lane's own registers; only the first/last row of each wave
+//                  crosses waves through a small double-buffered LDS slot.
+//                  Halo cells outside the domain are loaded by reflection, so
+//                  the no-flux boundary costs nothing inside the substep loop.
+#include "yh_internal.h"
+
+namespace yh {
+
+// ------------------------------------------------------------------ codes
+// reactionDiffusion.cu:156-169 with the centre coefficient folded in
+// (coeffx.y == coeffx.x + coeffx.z for every mask pattern).
+__global__ void k_codes(const uint8_t* __restrict__ solid, uint8_t* __restrict__ codes,
+                        int nx, int ny) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  const int c = i + nx * j;
+  const bool sc = solid[c] != 0;
+  if (!sc) {
+    codes[c] = kHole;
+    return;
+  }
+  const bool sw = solid[mirror(i - 1, nx) + nx * j] != 0;
+  const bool se = solid[mirror(i + 1, nx) + nx * j] != 0;
+  const bool sn = solid[i + nx * mirror(j + 1, ny)] != 0;
+  const bool ss = solid[i + nx * mirror(j - 1, ny)] != 0;
+  const int cW = (sw && se) ? 1 : (sw ? 2 : 0);
+  const int cE = (sw && se) ? 1 : (se ? 2 : 0);
+  const int cN = (sn && ss) ? 1 : (sn ? 2 : 0);
+  const int cS = (sn && ss) ? 1 : (ss ? 2 : 0);
+  codes[c] = (uint8_t)(cW | (cE << 2) | (cN << 4) | (cS << 6));
+}
+
+hipError_t launch_codes(const uint8_t* solid, uint8_t* codes, int nx, int ny, hipStream_t s) {
+  dim3 b(64, 4), g((nx + 63) / 64, (ny + 3) / 4);
+  hipLaunchKernelGGL(k_codes, g, b, 0, s, solid, codes, nx, ny);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ direct
+template <int MODEL, bool MASKED>
+__global__ __launch_bounds__(256) void k_direct(const KParams k, CFields in, Fields out,
+                                                Fields vt, const uint8_t* __restrict__ solid) {
+  using M = Model<MODEL>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= k.nx || j >= k.ny) return;
+  const int nx = k.nx, ny = k.ny;
+  const int c = i + nx * j;
+  const int W = mirror(i - 1, nx) + nx * j, E = mirror(i + 1, nx) + nx * j;
+  const int N = i + nx * mirror(j + 1, ny), S = i + nx * mirror(j - 1, ny);
+  float cW = 1.f, cE = 1.f, cN = 1.f, cS = 1.f;
+  bool sc = true;
+  if (MASKED) {
+    sc = solid[c] != 0;
+    const bool sw = solid[W] != 0, se = solid[E] != 0, sn = solid[N] != 0, ss = solid[S] != 0;
+    cW = (sw && se && sc) ? 1.f : ((sw && sc) ? 2.f : 0.f);
+    cE = (sw && se && sc) ? 1.f : ((se && sc) ? 2.f : 0.f);
+    cN = (sn && ss && sc) ? 1.f : ((sn && sc) ? 2.f : 0.f);
+    cS = (sn && ss && sc) ? 1.f : ((ss && sc) ? 2.f : 0.f);
+  }
+  float x[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < M::NV; ++a) x[a] = in.f[a][c];
+#pragma unroll
+  for (int a = 0; a < M::ND; ++a) {
+    const float* f = in.f[a];
+    const float fc = x[a];
+    const float lx = cW * (f[W] - fc) + cE * (f[E] - fc);
+    const float ly = cN * (f[N] - fc) + cS * (f[S] - fc);
+    d[a] = k.rx * lx + k.ry * ly;
+  }
+  float r[3] = {0.f, 0.f, 0.f};
+  M::react(k, x[0], x[1], x[2], r[0], r[1], r[2]);
+  // rhs = diffusion - dt*I ; new = old + tc*rhs  (reactionDiffusion.cu:498-513)
+  const float scale[3] = {1.0f, MODEL == M2V ? k.rs : 0.0f, 0.0f};
+#pragma unroll
+  for (int a = 0; a < M::NV; ++a) {
+    const float inc = k.tc * scale[a] * d[a] + r[a];
+    out.f[a][c] = sc ? x[a] + inc : 0.0f;
+    if (vt.f[0]) vt.f[a][c] = sc ? inc / k.tcdt : 0.0f;  // rhs/dt (:529-530,551-552)
+  }
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_direct_codes(const KParams k, CFields in, Fields out,
+                                                      const uint8_t* __restrict__ codes) {
+  using M = Model<MODEL>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= k.nx || j >= k.ny) return;
+  const int nx = k.nx, ny = k.ny;
+  const int c = i + nx * j;
+  const int W = mirror(i - 1, nx) + nx * j, E = mirror(i + 1, nx) + nx * j;
+  const int N = i + nx * mirror(j + 1, ny), S = i + nx * mirror(j - 1, ny);
+  float cW = 1.f, cE = 1.f, cN = 1.f, cS = 1.f;
+  bool sc = true;
+  if (codes) {
+    const uint32_t b = codes[c];
+    sc = b != kHole;
+    const uint32_t q = sc ? b : 0u;
+    cW = (float)(q & 3u);
+    cE = (float)((q >> 2) & 3u);
+    cN = (float)((q >> 4) & 3u);
+    cS = (float)((q >> 6) & 3u);
+  }
+  float x[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < M::NV; ++a) x[a] = in.f[a][c];
+#pragma unroll
+  for (int a = 0; a < M::ND; ++a) {
+    const float* f = in.f[a];
+    const float fc = x[a];
+    d[a] = k.rx * (cW * (f[W] - fc) + cE * (f[E] - fc)) + k.ry * (cN * (f[N] - fc) + cS * (f[S] - fc));
+  }
+  float r[3] = {0.f, 0.f, 0.f};
+  M::react(k, x[0], x[1], x[2], r[0], r[1], r[2]);
+  const float scale[3] = {1.0f, MODEL == M2V ? k.rs : 0.0f, 0.0f};
+#pragma unroll
+  for (int a = 0; a < M::NV; ++a) out.f[a][c] = sc ? x[a] + (k.tc * scale[a] * d[a] + r[a]) : 0.0f;
+}
+
+hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fields vt,
+                         const uint8_t* solid, hipStream_t s) {
+  dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
+  if (model == M2V) {
+    if (solid) hipLaunchKernelGGL((k_direct<M2V, true>), g, b, 0, s, k, in, out, vt, solid);
+    else hipLaunchKernelGGL((k_direct<M2V, false>), g, b, 0, s, k, in, out, vt, solid);
+  } else {
+    if (solid) hipLaunchKernelGGL((k_direct<MFK, true>), g, b, 0, s, k, in, out, vt, solid);
+    else hipLaunchKernelGGL((k_direct<MFK, false>), g, b, 0, s, k, in, out, vt, solid);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields out,
+                               const uint8_t* codes, hipStream_t s) {
+  dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
+  if (model == M2V) hipLaunchKernelGGL((k_direct_codes<M2V>), g, b, 0, s, k, in, out, codes);
+  else hipLaunchKernelGGL((k_direct_codes<MFK>), g, b, 0, s, k, in, out, codes);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- tiled
+__device__ __forceinline__ float lane_from(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
+// Workgroup -> tile with consecutive tiles on the same XCD: workgroups are
+// dispatched round-robin over the 8 XCDs, so XCD x receives a contiguous band
+// of tiles and neighbouring halo rows hit the same L2.
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
+  const int q = ntiles >> 3, rem = ntiles & 7;
+  const int x = bid & 7, idx = bid >> 3;
+  return x * q + (x < rem ? x : rem) + idx;
+}
+
+template <int MODEL, bool MASKED, int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, Fields out,
+                                                   const uint8_t* __restrict__ codes, int K,
+                                                   int tiles_x, int ntiles) {
+  using M = Model<MODEL>;
+  constexpr int NV = M::NV, ND = M::ND;
+  __shared__ float xch[2][2][NW][ND][64];  // [buffer][first/last row][wave][var][lane]
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tile = xcd_tile(blockIdx.x, ntiles);
+  const int ty = tile / tiles_x;
+  const int tx = tile - ty * tiles_x;
+  const int wout = 64 - 2 * K;
+  const int hout = NW * R - 2 * K;
+  const int nx = k.nx, ny = k.ny;
+  const int gx = tx * wout - K + lane;
+  const int gy0 = ty * hout - K + wave * R;
+  const int sx = reflect_clamp(gx, nx);
+  const bool flip_x = (gx < 0) || (gx >= nx);
+
+  float X[NV][R];
+  float cw[R], ce[R], cn[R], cs[R];
+  uint32_t hole = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int gy = gy0 + r;
+    const int sy = reflect_clamp(gy, ny);
+    const size_t idx = (size_t)sy * nx + sx;
+#pragma unroll
+    for (int a = 0; a < NV; ++a) X[a][r] = in.f[a][idx];
+    if (MASKED) {
+      const uint32_t b = codes[idx];
+      const bool flip_y = (gy < 0) || (gy >= ny);
+      uint32_t q = (b == kHole) ? 0u : b;
+      hole |= (b == kHole ? 1u : 0u) << r;
+      uint32_t w2 = q & 3u, e2 = (q >> 2) & 3u, n2 = (q >> 4) & 3u, s2 = (q >> 6) & 3u;
+      if (flip_x) { const uint32_t t = w2; w2 = e2; e2 = t; }
+      if (flip_y) { const uint32_t t = n2; n2 = s2; s2 = t; }
+      cw[r] = k.rx * (float)w2;
+      ce[r] = k.rx * (float)e2;
+      cn[r] = k.ry * (float)n2;
+      cs[r] = k.ry * (float)s2;
+    }
+  }
+
+  const int addr_w = ((lane + 63) & 63) << 2;
+  const int addr_e = ((lane + 1) & 63) << 2;
+  const float scale1 = (MODEL == M2V) ? k.rs : 0.0f;
+
+  for (int s = 0; s < K; ++s) {
+    const int b = s & 1;
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+      xch[b][0][wave][a][lane] = X[a][0];
+      xch[b][1][wave][a][lane] = X[a][R - 1];
+    }
+    __syncthreads();
+    float south[ND], north[ND];
+#pragma unroll
+    for (int a = 0; a < ND; ++a) {
+      south[a] = wave > 0 ? xch[b][1][wave - 1][a][lane] : X[a][0];
+      north[a] = wave < NW - 1 ? xch[b][0][wave + 1][a][lane] : X[a][R - 1];
+    }
+    float prev[ND];
+#pragma unroll
+    for (int a = 0; a < ND; ++a) prev[a] = south[a];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float lap[ND];
+#pragma unroll
+      for (int a = 0; a < ND; ++a) {
+        const float c = X[a][r];
+        const float w = lane_from(c, addr_w);
+        const float e = lane_from(c, addr_e);
+        const float sv = prev[a];
+        const float nv = (r < R - 1) ? X[a][r + 1] : north[a];
+        if (MASKED) {
+          lap[a] = cw[r] * (w - c) + ce[r] * (e - c) + cn[r] * (nv - c) + cs[r] * (sv - c);
+        } else {
+          lap[a] = k.rx * (w + e - 2.0f * c) + k.ry * (nv + sv - 2.0f * c);
+        }
+        prev[a] = c;
+      }
+      float rr[3] = {0.f, 0.f, 0.f};
+      M::react(k, X[0][r], X[1][r], NV > 2 ? X[NV - 1][r] : 0.0f, rr[0], rr[1], rr[2]);
+      const bool h = MASKED && ((hole >> r) & 1u);
+      float nu = X[0][r] + (k.tc * lap[0] + rr[0]);
+      if (MODEL == M2V) {
+        float nv2 = X[1][r] + (k.tc * scale1 * lap[ND - 1] + rr[1]);
+        X[1][r] = h ? 0.0f : nv2;
+      } else {
+        const float nv2 = X[1][r] + rr[1];
+        const float nw2 = X[NV - 1][r] + rr[2];
+        X[1][r] = h ? 0.0f : nv2;
+        X[NV - 1][r] = h ? 0.0f : nw2;
+      }
+      X[0][r] = h ? 0.0f : nu;
+    }
+  }
+
+  if (lane >= K && lane < 64 - K && gx < nx) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = wave * R + r;
+      const int gy = gy0 + r;
+      if (lr >= K && lr < NW * R - K && gy < ny) {
+        const size_t idx = (size_t)gy * nx + gx;
+#pragma unroll
+        for (int a = 0; a < NV; ++a) out.f[a][idx] = X[a][r];
+      }
+    }
+  }
+}
+
+bool tiled_supported(int rows, int waves) {
+  return (rows == 4 || rows == 8 || rows == 16) && (waves == 4 || waves == 8);
+}
+
+template <int MODEL, bool MASKED, int R, int NW>
+static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out,
+                                 const uint8_t* codes, hipStream_t s) {
+  const int wout = 64 - 2 * K, hout = NW * R - 2 * K;
+  if (wout <= 0 || hout <= 0) return hipErrorInvalidValue;
+  const int tiles_x = (k.nx + wout - 1) / wout;
+  const int tiles_y = (k.ny + hout - 1) / hout;
+  const int ntiles = tiles_x * tiles_y;
+  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW>), dim3(ntiles), dim3(64 * NW), 0, s, k, in,
+                     out, codes, K, tiles_x, ntiles);
+  return hipGetLastError();
+}
+
+template <int MODEL, bool MASKED>
+static hipError_t launch_tiled_m(int rows, int waves, int K, const KParams& k, CFields in,
+                                 Fields out, const uint8_t* codes, hipStream_t s) {
+#define YH_TILE_CASE(RR, WW) \
+  if (rows == RR && waves == WW) return launch_tiled_t<MODEL, MASKED, RR, WW>(K, k, in, out, codes, s);
+  YH_TILE_CASE(4, 4)
+  YH_TILE_CASE(8, 4)
+  YH_TILE_CASE(16, 4)
+  YH_TILE_CASE(4, 8)
+  YH_TILE_CASE(8, 8)
+  YH_TILE_CASE(16, 8)
+#undef YH_TILE_CASE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_tiled(int model, bool masked, int rows, int waves, int K, const KParams& k,
+                        CFields in, Fields out, const uint8_t* codes, hipStream_t s) {
+  if (model == M2V) {
+    return masked ? launch_tiled_m<M2V, true>(rows, waves, K, k, in, out, codes, s)
+                  : launch_tiled_m<M2V, false>(rows, waves, K, k, in, out, codes, s);
+  }
+  return masked ? launch_tiled_m<MFK, true>(rows, waves, K, k, in, out, codes, s)
+                : launch_tiled_m<MFK, false>(rows, waves, K, k, in, out, codes, s);
+}
+
+// -------------------------------------------------------------------- tips
+// spiralTip_kernel + tipRecordPlane (tipTracker.cu:20-241) in FP64 from the
+// FP32 fields, appending {x, y, 0, 0, t} and the key 2*cell+root.
+__device__ __forceinline__ void tip_record(int i, int j, float tx, float ty, float t, int key,
+                                           yh_tip* tips, int* keys, int* count, int cap) {
+  if ((tx > 0.0f) && (tx < 1.0f) && (ty > 0.0f) && (ty < 1.0f)) {
+    const int slot = atomicAdd(count, 1);
+    if (slot < cap) {
+      yh_tip d;
+      d.x = (float)i + tx;
+      d.y = (float)j + ty;
+      d.vx = 0.0f;
+      d.vy = 0.0f;
+      d.t = t;
+      tips[slot] = d;
+      keys[slot] = key;
+    }
+  }
+}
+
+__global__ void k_tips(int nx, int ny, int masked_disk, int r2, double Uth,
+                       const float* __restrict__ past, const float* __restrict__ present, float t,
+                       yh_tip* tips, int* keys, int* count, int cap) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  bool inside;
+  if (masked_disk) {
+    const int ic = i - nx / 2, jc = j - ny / 2;
+    inside = (ic * ic + jc * jc) < r2;
+  } else {
+    inside = (i >= 1) && (i < nx - 2) && (j >= 1) && (j < ny - 2);
+  }
+  if (!inside) return;
+  const int s0 = i + nx * j;
+  const int sx = (i < nx - 1) ? s0 + 1 : s0;
+  const int sy = (j < ny - 1) ? s0 + nx : s0;
+  const int sxy = ((j < ny - 1) && (i < nx - 1)) ? s0 + nx + 1 : s0;
+  const double x1 = present[s0], x2 = present[sx], x4 = present[sy], x3 = present[sxy];
+  const double y1 = past[s0], y2 = past[sx], y4 = past[sy], y3 = past[sxy];
+  const double x3y1 = x3 * y1, x4y1 = x4 * y1, x3y2 = x3 * y2, x4y2 = x4 * y2;
+  const double x1y3 = x1 * y3, x2y3 = x2 * y3, x1y4 = x1 * y4, x2y4 = x2 * y4;
+  const double x2y1 = x2 * y1, x1y2 = x1 * y2, x4y3 = x4 * y3, x3y4 = x3 * y4;
+  const double den1 = 2.0 * (x3y1 - x4y1 - x3y2 + x4y2 - x1y3 + x2y3 + x1y4 - x2y4);
+  const double den2 = 2.0 * (x2y1 - x3y1 - x1y2 + x4y2 + x1y3 - x4y3 - x2y4 + x3y4);
+  const double ctn1 = x1 - x2 + x3 - x4 - y1 + y2 - y3 + y4;
+  const double ctn2 = x3y1 - 2.0 * x4y1 + x4y2 - x1y3 + 2.0 * x1y4 - x2y4;
+  const double disc = sqrt(4.0 * (x3y1 - x3y2 - x4y1 + x4y2 - x1y3 + x1y4 + x2y3 - x2y4)
+                           * (x4y1 - x1y4 + Uth * (x1 - x4 - y1 + y4))
+                           + (-ctn2 + Uth * ctn1) * (-ctn2 + Uth * ctn1));
+  const double px = ctn2 - Uth * ctn1;
+  const double py = Uth * ctn1 - x3y1 + x4y2 + x1y3 - x2y4 + 2.0 * (x2y1 - x1y2);
+  if (!masked_disk && !(disc >= 0.0)) return;
+  tip_record(i, j, (float)((px + disc) / den1), (float)((py + disc) / den2), t, 2 * s0, tips,
+             keys, count, cap);
+  tip_record(i, j, (float)((px - disc) / den1), (float)((py - disc) / den2), t, 2 * s0 + 1, tips,
+             keys, count, cap);
+}
+
+hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int tipOffsetY,
+                       double Uth, const float* past, const float* present, double time,
+                       yh_tip* tips, int* keys, int* count, int cap, hipStream_t s) {
+  dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
+  hipLaunchKernelGGL(k_tips, g, b, 0, s, k.nx, k.ny, masked_disk, tipOffsetX * tipOffsetY, Uth,
+                     past, present, (float)time, tips, keys, count, cap);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------- finite
+__global__ void k_finite(const float* __restrict__ a, const float* __restrict__ b,
+                         const float* __restrict__ c, size_t n, int* flag) {
+  bool bad = false;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    bad |= !isfinite(a[t]);
+    if (b) bad |= !isfinite(b[t]);
+    if (c) bad |= !isfinite(c[t]);
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+hipError_t launch_finite(const float* const* f, int nf, size_t n, int* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_finite, dim3(1024), dim3(256), 0, s, f[0], nf > 1 ? f[1] : nullptr,
+                     nf > 2 ? f[2] : nullptr, n, flag);
+  return hipGetLastError();
+}
+
+}  // namespace yh
