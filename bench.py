@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--holes", type=int, default=200)
     ap.add_argument("--model", choices=["fk", "2v"], default="fk")
     ap.add_argument("--tip-every", type=int, default=100)
-    ap.add_argument("--cfg", default="auto", help="auto or K,rows,waves")
+    ap.add_argument("--cfg", default="auto", help="auto or K,rows,waves,cols")
     ap.add_argument("--kernel-launches", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -104,11 +104,11 @@ def main():
         best = autotune.tune_plan(pl, fields, steps=400, log=log)
         cfg = best["cfg"]
     else:
-        K, r, w = (int(x) for x in a.cfg.split(","))
-        cfg = (T.YH_KERNEL_TILED, K, r, w)
+        K, r, w, c = (int(x) for x in a.cfg.split(","))
+        cfg = (T.YH_KERNEL_TILED, K, r, w, c)
     pl.set_kernel(*cfg)
     pl.set_state(*fields)
-    log(f"kernel cfg variant={cfg[0]} K={cfg[1]} rows={cfg[2]} waves={cfg[3]}")
+    log(f"kernel cfg variant={cfg[0]} K={cfg[1]} rows={cfg[2]} waves={cfg[3]} cols={cfg[4]}")
 
     def run(nsteps):
         done = 0
@@ -186,7 +186,7 @@ def main():
                         f"model, {a.holes} holes, {a.steps} steps, tips every {a.tip_every}",
             "nx": nx, "ny": ny, "holes": a.holes, "dt_ms": 0.02, "dx_cm": 0.025,
             "D_cm2_per_ms": 0.001, "kernel": {"name": kname, "K": cfg[1], "rows": cfg[2],
-                                              "waves": cfg[3]},
+                                              "waves": cfg[3], "cols": cfg[4]},
         },
         "roofline": {
             "bound": "hbm",

@@ -135,7 +135,8 @@ typedef struct yh_kernel_cfg {
   int variant;   /* YH_KERNEL_* */
   int substeps;  /* k fused time steps per launch (TILED), 1..16 */
   int rows;      /* rows held in registers per thread (TILED): 4, 8 or 16 */
-  int waves;     /* wavefronts per workgroup (TILED): 4 or 8 */
+  int waves;     /* wavefronts per workgroup (TILED): 4, 8 or 16 */
+  int cols;      /* columns per lane (TILED): 1 or 2 */
 } yh_kernel_cfg;
 
 /* Creates a plan: checks parameters, allocates ping-pong state on the current
@@ -180,6 +181,37 @@ int yh_set_option(yh_plan* plan, int option, int64_t value);
  * afterwards), installs the fastest, and caches it per (nx, ny, model,
  * masked) for the process.  best / best_ms (ms per step) may be NULL. */
 int yh_autotune(yh_plan* plan, int steps, void* stream, yh_kernel_cfg* best, float* best_ms);
+
+/* ----------------------------------------------------- symmetry reduction */
+/* State of a co-moving-frame run (main.cu:156-161 c, phi; integrals of
+ * integralTrapz.cu:97-177; disk centre of symmetryReduction.cu:95-107). */
+typedef struct yh_sr_info {
+  double c[3];      /* translation (cx, cy) and rotation c_theta of the last solve */
+  double phi[3];    /* accumulated frame shift and angle */
+  double I[12];     /* the 12 disk inner products of the last step */
+  int cx, cy;       /* disk centre offset from (nx/2, ny/2) */
+  int64_t count;    /* SR steps taken */
+  int status;       /* bit 0: a solve was singular */
+  int ntips;        /* tips found in the last step */
+} yh_sr_info;
+
+/* Starts a symmetry-reduction run on the plan's current state (requires a
+ * plan created with reduceSym = 1, so dt is the halved SR step, main.cu:148).
+ * tip0 is the disk centre used at count 0 (tipx0_d, main.cu:373-376). */
+int yh_sr_begin(yh_plan* plan, float tip0x, float tip0y, void* stream);
+/* nsteps SR steps (display(), main.cu:894-953): RD, tips, disk inner
+ * products, 3x3 solve, BFECC advection, phi += dt*c; all on the device.
+ * c_phi_out (may be NULL) receives 6 doubles per step: c and phi before that
+ * step's solve (clist/philist, main.cu:902-903).  Tips of every step go to the
+ * plan's tip log. */
+int yh_sr_step(yh_plan* plan, int nsteps, void* stream, double* c_phi_out);
+/* Replay steady SR steps from one captured hipGraph (default on). */
+int yh_sr_set_graph(yh_plan* plan, int on);
+int yh_sr_info_get(yh_plan* plan, yh_sr_info* out);
+/* solve_matrix (symmetryReduction.cu:329-420) on the host: c from phi and the
+ * 12 integrals by the reference's unpivoted elimination; YH_ESINGULAR when the
+ * result is not finite. */
+int yh_solve_matrix(const double phi[3], const double I[12], double c[3]);
 
 #ifdef __cplusplus
 }

@@ -141,3 +141,17 @@ def test_cross_field_layout():
     U, V = u.reshape(32, 64), v.reshape(32, 64)
     assert np.all(U[:, :8] == 1) and np.all(U[:, 8:] == 0)  # main.cu:606-611
     assert np.all(V[16:, :] == 1) and np.all(V[:16, :] == 0)  # main.cu:613-618
+
+
+def test_solve_matrix_host_matches_oracle():
+    import ctypes as C
+    from yolohtli_amd._lib import lib
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        I = rng.normal(size=12) + np.array([3, 0, 0, 0, 3, 0, 0, 0, 3, 0, 0, 0])
+        phi = rng.normal(size=3)
+        c = np.zeros(3)
+        rc = lib().yh_solve_matrix(phi.ctypes.data_as(C.c_void_p), I.ctypes.data_as(C.c_void_p),
+                                   c.ctypes.data_as(C.c_void_p))
+        assert rc == 0
+        np.testing.assert_allclose(c, O.solve_matrix(phi, I), rtol=1e-13)

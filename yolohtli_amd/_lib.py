@@ -7,7 +7,7 @@ through the HIP library, and a missing library raises immediately.
 import ctypes as C
 import os
 
-from .abi_types import YhDerived, YhKernelCfg, YhParams, YhTip, YhFkParams  # noqa: F401
+from .abi_types import YhDerived, YhKernelCfg, YhParams, YhTip, YhFkParams, YhSrInfo  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libyolohtli_amd.so")
@@ -49,6 +49,11 @@ PROTOTYPES = {
     "yh_check_finite": (I, [P, P]),
     "yh_set_option": (I, [P, I, C.c_int64]),
     "yh_autotune": (I, [P, I, P, C.POINTER(YhKernelCfg), C.POINTER(C.c_float)]),
+    "yh_sr_begin": (I, [P, C.c_float, C.c_float, P]),
+    "yh_sr_step": (I, [P, I, P, P]),
+    "yh_sr_set_graph": (I, [P, I]),
+    "yh_sr_info_get": (I, [P, C.POINTER(YhSrInfo)]),
+    "yh_solve_matrix": (I, [P, P, P]),
 }
 
 

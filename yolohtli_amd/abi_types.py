@@ -58,7 +58,13 @@ class YhDerived(C.Structure):
 
 class YhKernelCfg(C.Structure):
     _fields_ = [("variant", C.c_int), ("substeps", C.c_int), ("rows", C.c_int),
-                ("waves", C.c_int)]
+                ("waves", C.c_int), ("cols", C.c_int)]
+
+
+class YhSrInfo(C.Structure):
+    _fields_ = [("c", C.c_double * 3), ("phi", C.c_double * 3), ("I", C.c_double * 12),
+                ("cx", C.c_int), ("cy", C.c_int), ("count", C.c_int64), ("status", C.c_int),
+                ("ntips", C.c_int)]
 
 
 class YhTip(C.Structure):

@@ -14,8 +14,9 @@ def tune_plan(plan, fields=None, steps=400, stream=None, log=None):
     ms = C.c_float()
     check(lib().yh_autotune(plan._h, int(steps), _stream_handle(stream), C.byref(cfg),
                             C.byref(ms)), "yh_autotune")
-    out = {"cfg": (cfg.variant, cfg.substeps, cfg.rows, cfg.waves), "ms_per_step": ms.value}
+    out = {"cfg": (cfg.variant, cfg.substeps, cfg.rows, cfg.waves, cfg.cols),
+           "ms_per_step": ms.value}
     if log:
         log(f"autotune: variant={cfg.variant} K={cfg.substeps} rows={cfg.rows} "
-            f"waves={cfg.waves}: {ms.value * 1e3:.3f} us/step")
+            f"waves={cfg.waves} cols={cfg.cols}: {ms.value * 1e3:.3f} us/step")
     return out

@@ -31,35 +31,42 @@ int hip_fail(hipError_t e, const char* what) {
 int nvars_of(int model) { return model == YH_MODEL_3V_FK ? 3 : 2; }
 
 void make_kparams(const yh_params& p, const yh_derived& d, KParams* k) {
+  const double tcdt = p.tc * d.dt;
   k->nx = p.nx;
   k->ny = p.ny;
   k->rx = (float)d.rx;
   k->ry = (float)d.ry;
+  k->trx = (float)(p.tc * d.rx);
+  k->try_ = (float)(p.tc * d.ry);
   k->dt = (float)d.dt;
   k->tc = (float)p.tc;
-  k->tcdt = (float)(p.tc * d.dt);
+  k->tcdt = (float)tcdt;
   k->rs = p.gateDiff ? (float)p.rscale : 0.0f;
-  k->mu = (float)p.mu;
+  // 2V: -tcdt*I_sum = u*(tcdt*mu*(1-u)*(u-alpha) - tcdt*v)
+  //     -tcdt*I_v   = tcdt*eps*delta*(u-gamma)*(beta-u) - tcdt*eps*v - tcdt*eps*theta
   k->alpha = (float)p.alpha;
   k->beta = (float)p.beta;
   k->gamma = (float)p.gamma;
-  k->delta = (float)p.delta;
-  k->eps = (float)p.eps;
-  k->theta = (float)p.theta;
+  k->a2_A = (float)(tcdt * p.mu);
+  k->a2_B = (float)tcdt;
+  k->a2_E = (float)(tcdt * p.eps * p.delta);
+  k->a2_F = (float)(tcdt * p.eps);
+  k->a2_G = (float)(tcdt * p.eps * p.theta);
   const yh_fk_params& f = p.fk;
-  k->i_td = (float)(1.0 / f.tau_d);
-  k->i_tr = (float)(1.0 / f.tau_r);
-  k->i_2tsi = (float)(1.0 / (2.0 * f.tau_si));
-  k->i_t0 = (float)(1.0 / f.tau_0);
-  k->i_tvp = (float)(1.0 / f.tau_vp);
-  k->i_tv1m = (float)(1.0 / f.tau_v1m);
-  k->i_tv2m = (float)(1.0 / f.tau_v2m);
-  k->i_twp = (float)(1.0 / f.tau_wp);
-  k->i_twm = (float)(1.0 / f.tau_wm);
   k->u_c = (float)f.u_c;
   k->u_v = (float)f.u_v;
-  k->u_csi = (float)f.u_csi;
-  k->kk = (float)f.k;
+  k->fk_A1 = (float)(tcdt / f.tau_d);
+  k->fk_A2 = (float)(tcdt / f.tau_r);
+  k->fk_A3 = (float)(tcdt / f.tau_0);
+  k->fk_A4 = (float)(tcdt / f.tau_si);
+  const double log2e = 1.4426950408889634;
+  k->fk_b = (float)(2.0 * f.k * log2e);
+  k->fk_bu = (float)(2.0 * f.k * log2e * f.u_csi);
+  k->fk_B1 = (float)(tcdt / f.tau_vp);
+  k->fk_B2 = (float)(tcdt / f.tau_v1m);
+  k->fk_B3 = (float)(tcdt / f.tau_v2m);
+  k->fk_C1 = (float)(tcdt / f.tau_wp);
+  k->fk_C2 = (float)(tcdt / f.tau_wm);
   k->hw = (float)p.heaviside_width;
 }
 

@@ -19,18 +19,18 @@ std::map<Key, std::pair<yh_kernel_cfg, float>> g_cache;
 
 std::vector<yh_kernel_cfg> candidates(const yh_plan* pl) {
   std::vector<yh_kernel_cfg> c;
-  c.push_back({YH_KERNEL_DIRECT, 1, 0, 0});
+  c.push_back({YH_KERNEL_DIRECT, 1, 0, 0, 1});
   const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
-  const int Ks[] = {1, 2, 3, 4, 6, 8, 12};
-  const int shapes[][2] = {{4, 8}, {8, 4}, {8, 8}, {16, 4}, {16, 8}, {4, 4}};
+  const int Ks[] = {1, 2, 4, 6, 8, 12};
+  const int shapes[][3] = {{4, 8, 1}, {8, 8, 1}, {4, 16, 1}, {8, 16, 1}, {16, 8, 1},
+                           {4, 8, 2}, {8, 8, 2}, {4, 16, 2}};
   for (int K : Ks) {
     if (K > kmax) continue;
     for (auto& s : shapes) {
-      const int rows = s[0], waves = s[1];
-      if (2 * K >= 64 || 2 * K >= rows * waves) continue;
-      // FK keeps three fields per row in registers: 16 rows spills occupancy.
-      if (pl->p.model == YH_MODEL_3V_FK && rows == 16) continue;
-      c.push_back({YH_KERNEL_TILED, K, rows, waves});
+      const int rows = s[0], waves = s[1], cols = s[2];
+      if (2 * K >= 64 * cols || 2 * K >= rows * waves) continue;
+      if (!yh::tiled_supported(rows, waves, cols)) continue;
+      c.push_back({YH_KERNEL_TILED, K, rows, waves, cols});
     }
   }
   return c;

@@ -15,9 +15,10 @@ yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
   yh_kernel_cfg c = pl->cfg;
   if (c.variant == YH_KERNEL_AUTO) {
     c.variant = YH_KERNEL_TILED;
-    c.substeps = 4;
+    c.substeps = 8;
     c.rows = 8;
-    c.waves = 8;
+    c.waves = 16;
+    c.cols = 1;
   }
   if (c.variant == YH_KERNEL_TILED) {
     const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
@@ -44,7 +45,7 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
       CFields ci;
       Fields co;
       fields(pl->cur, &ci, &co);
-      YH_HIP(launch_tiled(pl->p.model, masked, c.rows, c.waves, K, pl->k, ci, co,
+      YH_HIP(launch_tiled(pl->p.model, masked, c.rows, c.waves, c.cols, K, pl->k, ci, co,
                           masked ? pl->codes_d : nullptr, s));
       pl->cur ^= 1;
       remaining -= K;
@@ -179,9 +180,10 @@ int yh_set_kernel(yh_plan* pl, const yh_kernel_cfg* cfg) {
   if (!pl || !cfg) return YH_EINVAL;
   if (cfg->variant < YH_KERNEL_AUTO || cfg->variant > YH_KERNEL_TILED) return YH_EINVAL;
   if (cfg->variant == YH_KERNEL_TILED) {
-    if (!tiled_supported(cfg->rows, cfg->waves)) return YH_EINVAL;
+    if (!tiled_supported(cfg->rows, cfg->waves, cfg->cols)) return YH_EINVAL;
     if (cfg->substeps < 1 || cfg->substeps > 16) return YH_EINVAL;
-    if (2 * cfg->substeps >= 64 || 2 * cfg->substeps >= cfg->rows * cfg->waves) return YH_EINVAL;
+    if (2 * cfg->substeps >= 64 * cfg->cols || 2 * cfg->substeps >= cfg->rows * cfg->waves)
+      return YH_EINVAL;
   }
   pl->cfg = *cfg;
   return YH_OK;

@@ -25,7 +25,7 @@ struct yh_plan {
   int* tipcount_d = nullptr;
   int tipcap = 0;
   int* flag_d = nullptr;
-  yh_kernel_cfg cfg{YH_KERNEL_AUTO, 4, 8, 8};
+  yh_kernel_cfg cfg{YH_KERNEL_AUTO, 4, 8, 8, 1};
   int64_t count = 0;
   bool keep_prev = true;
   yh_sr* sr = nullptr;

@@ -54,6 +54,34 @@ hipError_t launch_codes(const uint8_t* solid, uint8_t* codes, int nx, int ny, hi
 }
 
 // ------------------------------------------------------------------ direct
+// One step, one thread per cell.  Diffusion in the coefficient form
+// sum_dir c_dir*(f_dir - f) with c in {0,1,2} (reactionDiffusion.cu:162-173);
+// c_W = c_E = c_N = c_S = 1 without a mask (:188-197).
+template <int MODEL>
+__device__ __forceinline__ void cell_update(const KParams& k, const float* const* f, int c, int W,
+                                            int E, int N, int S, float cW, float cE, float cN,
+                                            float cS, bool sc, float* xo, float* inc) {
+  using M = Model<MODEL>;
+  float x[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < M::NV; ++a) x[a] = f[a][c];
+  float lap[2] = {0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < M::ND; ++a) {
+    const float fc = x[a];
+    const float lx = cW * (f[a][W] - fc) + cE * (f[a][E] - fc);
+    const float ly = cN * (f[a][N] - fc) + cS * (f[a][S] - fc);
+    lap[a] = k.trx * lx + k.try_ * ly;
+  }
+  float r[3] = {0.f, 0.f, 0.f};
+  M::react(k, x[0], x[1], x[2], r[0], r[1], r[2]);
+  inc[0] = lap[0] + r[0];
+  inc[1] = (MODEL == M2V ? k.rs * lap[M::ND - 1] : 0.0f) + r[1];
+  inc[2] = r[2];
+#pragma unroll
+  for (int a = 0; a < M::NV; ++a) xo[a] = sc ? x[a] + inc[a] : 0.0f;
+}
+
 template <int MODEL, bool MASKED>
 __global__ __launch_bounds__(256) void k_direct(const KParams k, CFields in, Fields out,
                                                 Fields vt, const uint8_t* __restrict__ solid) {
@@ -75,26 +103,12 @@ __global__ __launch_bounds__(256) void k_direct(const KParams k, CFields in, Fie
     cN = (sn && ss && sc) ? 1.f : ((sn && sc) ? 2.f : 0.f);
     cS = (sn && ss && sc) ? 1.f : ((ss && sc) ? 2.f : 0.f);
   }
-  float x[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-  for (int a = 0; a < M::NV; ++a) x[a] = in.f[a][c];
-#pragma unroll
-  for (int a = 0; a < M::ND; ++a) {
-    const float* f = in.f[a];
-    const float fc = x[a];
-    const float lx = cW * (f[W] - fc) + cE * (f[E] - fc);
-    const float ly = cN * (f[N] - fc) + cS * (f[S] - fc);
-    d[a] = k.rx * lx + k.ry * ly;
-  }
-  float r[3] = {0.f, 0.f, 0.f};
-  M::react(k, x[0], x[1], x[2], r[0], r[1], r[2]);
-  // rhs = diffusion - dt*I ; new = old + tc*rhs  (reactionDiffusion.cu:498-513)
-  const float scale[3] = {1.0f, MODEL == M2V ? k.rs : 0.0f, 0.0f};
+  float xo[3], inc[3];
+  cell_update<MODEL>(k, in.f, c, W, E, N, S, cW, cE, cN, cS, sc, xo, inc);
 #pragma unroll
   for (int a = 0; a < M::NV; ++a) {
-    const float inc = k.tc * scale[a] * d[a] + r[a];
-    out.f[a][c] = sc ? x[a] + inc : 0.0f;
-    if (vt.f[0]) vt.f[a][c] = sc ? inc / k.tcdt : 0.0f;  // rhs/dt (:529-530,551-552)
+    out.f[a][c] = xo[a];
+    if (vt.f[0]) vt.f[a][c] = sc ? inc[a] / k.tcdt : 0.0f;  // rhs/dt (:529-530,551-552)
   }
 }
 
@@ -120,20 +134,10 @@ __global__ __launch_bounds__(256) void k_direct_codes(const KParams k, CFields i
     cN = (float)((q >> 4) & 3u);
     cS = (float)((q >> 6) & 3u);
   }
-  float x[3] = {0.f, 0.f, 0.f}, d[3] = {0.f, 0.f, 0.f};
+  float xo[3], inc[3];
+  cell_update<MODEL>(k, in.f, c, W, E, N, S, cW, cE, cN, cS, sc, xo, inc);
 #pragma unroll
-  for (int a = 0; a < M::NV; ++a) x[a] = in.f[a][c];
-#pragma unroll
-  for (int a = 0; a < M::ND; ++a) {
-    const float* f = in.f[a];
-    const float fc = x[a];
-    d[a] = k.rx * (cW * (f[W] - fc) + cE * (f[E] - fc)) + k.ry * (cN * (f[N] - fc) + cS * (f[S] - fc));
-  }
-  float r[3] = {0.f, 0.f, 0.f};
-  M::react(k, x[0], x[1], x[2], r[0], r[1], r[2]);
-  const float scale[3] = {1.0f, MODEL == M2V ? k.rs : 0.0f, 0.0f};
-#pragma unroll
-  for (int a = 0; a < M::NV; ++a) out.f[a][c] = sc ? x[a] + (k.tc * scale[a] * d[a] + r[a]) : 0.0f;
+  for (int a = 0; a < M::NV; ++a) out.f[a][c] = xo[a];
 }
 
 hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fields vt,
@@ -171,161 +175,233 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   return x * q + (x < rem ? x : rem) + idx;
 }
 
-template <int MODEL, bool MASKED, int R, int NW>
+// Region of one workgroup: (64*CW) columns x (NW*R) rows; lane L owns columns
+// CW*L .. CW*L+CW-1, wave w owns rows w*R .. w*R+R-1.  After K substeps the
+// inner (64*CW-2K) x (NW*R-2K) cells are exact; the outer ring of width K is
+// recomputed by the neighbouring workgroup.  In substep s only rows that stay
+// exact after it, s+1 <= row < NW*R-s-1, are computed (the test is uniform
+// over a wave).  Rows whose cells all have the regular stencil (c = 1,1,1,1)
+// take the unmasked formula; the others decode per-cell coefficient bytes.
+template <int MODEL, bool MASKED, int R, int NW, int CW>
 __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, Fields out,
                                                    const uint8_t* __restrict__ codes, int K,
                                                    int tiles_x, int ntiles) {
   using M = Model<MODEL>;
   constexpr int NV = M::NV, ND = M::ND;
-  __shared__ float xch[2][2][NW][ND][64];  // [buffer][first/last row][wave][var][lane]
+  constexpr int WR = 64 * CW;
+  constexpr int H = NW * R;
+  __shared__ float xch[2][2][NW][ND][WR];  // [buffer][first/last row][wave][var][column]
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int ty = tile / tiles_x;
   const int tx = tile - ty * tiles_x;
-  const int wout = 64 - 2 * K;
-  const int hout = NW * R - 2 * K;
+  const int wout = WR - 2 * K;
+  const int hout = H - 2 * K;
   const int nx = k.nx, ny = k.ny;
-  const int gx = tx * wout - K + lane;
+  const int gx0 = tx * wout - K + lane * CW;
   const int gy0 = ty * hout - K + wave * R;
-  const int sx = reflect_clamp(gx, nx);
-  const bool flip_x = (gx < 0) || (gx >= nx);
 
-  float X[NV][R];
-  float cw[R], ce[R], cn[R], cs[R];
-  uint32_t hole = 0;
+  int sx[CW];
+  bool flip_x[CW];
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    sx[c] = reflect_clamp(gx0 + c, nx);
+    flip_x[c] = (gx0 + c < 0) || (gx0 + c >= nx);
+  }
+
+  float X[NV][R][CW];
+  uint32_t cd[R][CW];   // coefficient bytes cW | cE<<8 | cN<<16 | cS<<24 (0 for holes)
+  uint32_t hole[CW];    // bit r: cell (r, c) is a hole
+  uint32_t irregular = 0;  // bit r: some cell of row r in this wave is not regular
+#pragma unroll
+  for (int c = 0; c < CW; ++c) hole[c] = 0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int gy = gy0 + r;
     const int sy = reflect_clamp(gy, ny);
-    const size_t idx = (size_t)sy * nx + sx;
+    const bool flip_y = (gy < 0) || (gy >= ny);
+    bool odd = false;
 #pragma unroll
-    for (int a = 0; a < NV; ++a) X[a][r] = in.f[a][idx];
-    if (MASKED) {
-      const uint32_t b = codes[idx];
-      const bool flip_y = (gy < 0) || (gy >= ny);
-      uint32_t q = (b == kHole) ? 0u : b;
-      hole |= (b == kHole ? 1u : 0u) << r;
-      uint32_t w2 = q & 3u, e2 = (q >> 2) & 3u, n2 = (q >> 4) & 3u, s2 = (q >> 6) & 3u;
-      if (flip_x) { const uint32_t t = w2; w2 = e2; e2 = t; }
-      if (flip_y) { const uint32_t t = n2; n2 = s2; s2 = t; }
-      cw[r] = k.rx * (float)w2;
-      ce[r] = k.rx * (float)e2;
-      cn[r] = k.ry * (float)n2;
-      cs[r] = k.ry * (float)s2;
+    for (int c = 0; c < CW; ++c) {
+      const size_t idx = (size_t)sy * nx + sx[c];
+#pragma unroll
+      for (int a = 0; a < NV; ++a) X[a][r][c] = in.f[a][idx];
+      if (MASKED) {
+        const uint32_t b = codes[idx];
+        const bool h = b == kHole;
+        const uint32_t q = h ? 0u : b;
+        uint32_t w2 = q & 3u, e2 = (q >> 2) & 3u, n2 = (q >> 4) & 3u, s2 = (q >> 6) & 3u;
+        if (flip_x[c]) { const uint32_t t = w2; w2 = e2; e2 = t; }
+        if (flip_y) { const uint32_t t = n2; n2 = s2; s2 = t; }
+        cd[r][c] = w2 | (e2 << 8) | (n2 << 16) | (s2 << 24);
+        hole[c] |= (h ? 1u : 0u) << r;
+        odd |= cd[r][c] != 0x01010101u;
+        if (h) {
+#pragma unroll
+          for (int a = 0; a < NV; ++a) X[a][r][c] = 0.0f;  // hole cells stay 0 (:521-522)
+        }
+      }
     }
+    if (MASKED && __any(odd)) irregular |= 1u << r;
   }
 
   const int addr_w = ((lane + 63) & 63) << 2;
   const int addr_e = ((lane + 1) & 63) << 2;
-  const float scale1 = (MODEL == M2V) ? k.rs : 0.0f;
 
   for (int s = 0; s < K; ++s) {
     const int b = s & 1;
 #pragma unroll
-    for (int a = 0; a < ND; ++a) {
-      xch[b][0][wave][a][lane] = X[a][0];
-      xch[b][1][wave][a][lane] = X[a][R - 1];
-    }
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        xch[b][0][wave][a][lane * CW + c] = X[a][0][c];
+        xch[b][1][wave][a][lane * CW + c] = X[a][R - 1][c];
+      }
     __syncthreads();
-    float south[ND], north[ND];
+    float south[ND][CW], north[ND][CW];
 #pragma unroll
-    for (int a = 0; a < ND; ++a) {
-      south[a] = wave > 0 ? xch[b][1][wave - 1][a][lane] : X[a][0];
-      north[a] = wave < NW - 1 ? xch[b][0][wave + 1][a][lane] : X[a][R - 1];
-    }
-    float prev[ND];
+    for (int a = 0; a < ND; ++a)
 #pragma unroll
-    for (int a = 0; a < ND; ++a) prev[a] = south[a];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      float lap[ND];
-#pragma unroll
-      for (int a = 0; a < ND; ++a) {
-        const float c = X[a][r];
-        const float w = lane_from(c, addr_w);
-        const float e = lane_from(c, addr_e);
-        const float sv = prev[a];
-        const float nv = (r < R - 1) ? X[a][r + 1] : north[a];
-        if (MASKED) {
-          lap[a] = cw[r] * (w - c) + ce[r] * (e - c) + cn[r] * (nv - c) + cs[r] * (sv - c);
-        } else {
-          lap[a] = k.rx * (w + e - 2.0f * c) + k.ry * (nv + sv - 2.0f * c);
-        }
-        prev[a] = c;
+      for (int c = 0; c < CW; ++c) {
+        south[a][c] = wave > 0 ? xch[b][1][wave - 1][a][lane * CW + c] : X[a][0][c];
+        north[a][c] = wave < NW - 1 ? xch[b][0][wave + 1][a][lane * CW + c] : X[a][R - 1][c];
       }
-      float rr[3] = {0.f, 0.f, 0.f};
-      M::react(k, X[0][r], X[1][r], NV > 2 ? X[NV - 1][r] : 0.0f, rr[0], rr[1], rr[2]);
-      const bool h = MASKED && ((hole >> r) & 1u);
-      float nu = X[0][r] + (k.tc * lap[0] + rr[0]);
-      if (MODEL == M2V) {
-        float nv2 = X[1][r] + (k.tc * scale1 * lap[ND - 1] + rr[1]);
-        X[1][r] = h ? 0.0f : nv2;
-      } else {
-        const float nv2 = X[1][r] + rr[1];
-        const float nw2 = X[NV - 1][r] + rr[2];
-        X[1][r] = h ? 0.0f : nv2;
-        X[NV - 1][r] = h ? 0.0f : nw2;
-      }
-      X[0][r] = h ? 0.0f : nu;
-    }
-  }
-
-  if (lane >= K && lane < 64 - K && gx < nx) {
+    float prev[ND][CW];
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < CW; ++c) prev[a][c] = south[a][c];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int lr = wave * R + r;
-      const int gy = gy0 + r;
-      if (lr >= K && lr < NW * R - K && gy < ny) {
-        const size_t idx = (size_t)gy * nx + gx;
+      const bool active = (lr >= s + 1) && (lr < H - s - 1);
+      float old[ND][CW];
 #pragma unroll
-        for (int a = 0; a < NV; ++a) out.f[a][idx] = X[a][r];
+      for (int a = 0; a < ND; ++a)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) old[a][c] = X[a][r][c];
+      if (active) {
+        float lap[ND][CW];
+#pragma unroll
+        for (int a = 0; a < ND; ++a) {
+          const float wl = lane_from(old[a][CW - 1], addr_w);  // west of column 0
+          const float er = lane_from(old[a][0], addr_e);       // east of column CW-1
+#pragma unroll
+          for (int c = 0; c < CW; ++c) {
+            const float ctr = old[a][c];
+            const float wv = c == 0 ? wl : old[a][c - 1];
+            const float ev = c == CW - 1 ? er : old[a][c + 1];
+            const float sv = prev[a][c];
+            const float nv = (r < R - 1) ? X[a][r + 1][c] : north[a][c];
+            if (MASKED && ((irregular >> r) & 1u)) {
+              const uint32_t q = cd[r][c];
+              const float cW = (float)(q & 0xFFu), cE = (float)((q >> 8) & 0xFFu);
+              const float cN = (float)((q >> 16) & 0xFFu), cS = (float)(q >> 24);
+              const float lx = __builtin_fmaf(cE, ev - ctr, cW * (wv - ctr));
+              const float ly = __builtin_fmaf(cS, sv - ctr, cN * (nv - ctr));
+              lap[a][c] = __builtin_fmaf(k.try_, ly, k.trx * lx);
+            } else {
+              const float lx = __builtin_fmaf(-2.0f, ctr, wv + ev);
+              const float ly = __builtin_fmaf(-2.0f, ctr, nv + sv);
+              lap[a][c] = __builtin_fmaf(k.try_, ly, k.trx * lx);
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+          float rr[3] = {0.f, 0.f, 0.f};
+          M::react(k, X[0][r][c], X[1][r][c], NV > 2 ? X[NV - 1][r][c] : 0.0f, rr[0], rr[1], rr[2]);
+          X[0][r][c] += lap[0][c] + rr[0];
+          if (MODEL == M2V) {
+            X[1][r][c] += __builtin_fmaf(k.rs, lap[ND - 1][c], rr[1]);
+          } else {
+            X[1][r][c] += rr[1];
+            X[NV - 1][r][c] += rr[2];
+          }
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < ND; ++a)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) prev[a][c] = old[a][c];
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wave * R + r;
+    const int gy = gy0 + r;
+    if (lr >= K && lr < H - K && gy < ny) {
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const int lc = lane * CW + c;
+        const int gx = gx0 + c;
+        if (lc >= K && lc < WR - K && gx < nx) {
+          const size_t idx = (size_t)gy * nx + gx;
+          const bool h = MASKED && ((hole[c] >> r) & 1u);
+#pragma unroll
+          for (int a = 0; a < NV; ++a) out.f[a][idx] = h ? 0.0f : X[a][r][c];
+        }
       }
     }
   }
 }
 
-bool tiled_supported(int rows, int waves) {
-  return (rows == 4 || rows == 8 || rows == 16) && (waves == 4 || waves == 8);
+bool tiled_supported(int rows, int waves, int cols) {
+  const bool rw = (rows == 4 && (waves == 4 || waves == 8 || waves == 16)) ||
+                  (rows == 8 && (waves == 4 || waves == 8 || waves == 16)) ||
+                  (rows == 16 && (waves == 4 || waves == 8));
+  // two columns per lane double the registers: 16 rows, or 8 rows in 16 waves, spill
+  return rw && (cols == 1 || (cols == 2 && rows * waves <= 64 && rows <= 8));
 }
 
-template <int MODEL, bool MASKED, int R, int NW>
+template <int MODEL, bool MASKED, int R, int NW, int CW>
 static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out,
                                  const uint8_t* codes, hipStream_t s) {
-  const int wout = 64 - 2 * K, hout = NW * R - 2 * K;
+  const int wout = 64 * CW - 2 * K, hout = NW * R - 2 * K;
   if (wout <= 0 || hout <= 0) return hipErrorInvalidValue;
   const int tiles_x = (k.nx + wout - 1) / wout;
   const int tiles_y = (k.ny + hout - 1) / hout;
   const int ntiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW>), dim3(ntiles), dim3(64 * NW), 0, s, k, in,
-                     out, codes, K, tiles_x, ntiles);
+  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW>), dim3(ntiles), dim3(64 * NW), 0, s, k,
+                     in, out, codes, K, tiles_x, ntiles);
   return hipGetLastError();
 }
 
 template <int MODEL, bool MASKED>
-static hipError_t launch_tiled_m(int rows, int waves, int K, const KParams& k, CFields in,
-                                 Fields out, const uint8_t* codes, hipStream_t s) {
-#define YH_TILE_CASE(RR, WW) \
-  if (rows == RR && waves == WW) return launch_tiled_t<MODEL, MASKED, RR, WW>(K, k, in, out, codes, s);
-  YH_TILE_CASE(4, 4)
-  YH_TILE_CASE(8, 4)
-  YH_TILE_CASE(16, 4)
-  YH_TILE_CASE(4, 8)
-  YH_TILE_CASE(8, 8)
-  YH_TILE_CASE(16, 8)
+static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KParams& k,
+                                 CFields in, Fields out, const uint8_t* codes, hipStream_t s) {
+#define YH_TILE_CASE(RR, WW, CC)                 \
+  if (rows == RR && waves == WW && cols == CC) \
+    return launch_tiled_t<MODEL, MASKED, RR, WW, CC>(K, k, in, out, codes, s);
+  YH_TILE_CASE(4, 4, 1)
+  YH_TILE_CASE(8, 4, 1)
+  YH_TILE_CASE(16, 4, 1)
+  YH_TILE_CASE(4, 8, 1)
+  YH_TILE_CASE(8, 8, 1)
+  YH_TILE_CASE(16, 8, 1)
+  YH_TILE_CASE(4, 16, 1)
+  YH_TILE_CASE(8, 16, 1)
+  YH_TILE_CASE(4, 4, 2)
+  YH_TILE_CASE(8, 4, 2)
+  YH_TILE_CASE(4, 8, 2)
+  YH_TILE_CASE(8, 8, 2)
+  YH_TILE_CASE(4, 16, 2)
 #undef YH_TILE_CASE
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_tiled(int model, bool masked, int rows, int waves, int K, const KParams& k,
-                        CFields in, Fields out, const uint8_t* codes, hipStream_t s) {
+hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
+                        const KParams& k, CFields in, Fields out, const uint8_t* codes,
+                        hipStream_t s) {
   if (model == M2V) {
-    return masked ? launch_tiled_m<M2V, true>(rows, waves, K, k, in, out, codes, s)
-                  : launch_tiled_m<M2V, false>(rows, waves, K, k, in, out, codes, s);
+    return masked ? launch_tiled_m<M2V, true>(rows, waves, cols, K, k, in, out, codes, s)
+                  : launch_tiled_m<M2V, false>(rows, waves, cols, K, k, in, out, codes, s);
   }
-  return masked ? launch_tiled_m<MFK, true>(rows, waves, K, k, in, out, codes, s)
-                : launch_tiled_m<MFK, false>(rows, waves, K, k, in, out, codes, s);
+  return masked ? launch_tiled_m<MFK, true>(rows, waves, cols, K, k, in, out, codes, s)
+                : launch_tiled_m<MFK, false>(rows, waves, cols, K, k, in, out, codes, s);
 }
 
 // -------------------------------------------------------------------- tips
@@ -349,12 +425,15 @@ __device__ __forceinline__ void tip_record(int i, int j, float tx, float ty, flo
 }
 
 __global__ void k_tips(int nx, int ny, int masked_disk, int r2, double Uth,
-                       const float* __restrict__ past, const float* __restrict__ present, float t,
-                       yh_tip* tips, int* keys, int* count, int cap) {
+                       const float* __restrict__ past, const float* __restrict__ present,
+                       float t_host, const long long* d_count, double dt, yh_tip* tips,
+                       int* keys, int* count, int cap) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= nx || j >= ny) return;
+  // physicalTime = dt*count (main.cu:885), from the device step counter when given
+  const float t = d_count ? (float)(dt * (double)(*d_count)) : t_host;
   bool inside;
   if (masked_disk) {
     const int ic = i - nx / 2, jc = j - ny / 2;
@@ -393,7 +472,18 @@ hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int ti
                        yh_tip* tips, int* keys, int* count, int cap, hipStream_t s) {
   dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
   hipLaunchKernelGGL(k_tips, g, b, 0, s, k.nx, k.ny, masked_disk, tipOffsetX * tipOffsetY, Uth,
-                     past, present, (float)time, tips, keys, count, cap);
+                     past, present, (float)time, (const long long*)nullptr, 0.0, tips, keys,
+                     count, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_tips_dev(const KParams& k, int masked_disk, int r2, double Uth,
+                           const float* past, const float* present, double dt,
+                           const long long* d_count, yh_tip* tips, int* keys, int* count, int cap,
+                           hipStream_t s) {
+  dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
+  hipLaunchKernelGGL(k_tips, g, b, 0, s, k.nx, k.ny, masked_disk, r2, Uth, past, present, 0.0f,
+                     d_count, dt, tips, keys, count, cap);
   return hipGetLastError();
 }
 

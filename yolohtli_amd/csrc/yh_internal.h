@@ -16,14 +16,21 @@ namespace yh {
 struct KParams {
   int nx, ny;
   float rx, ry;       // dt*Dxx/hx^2, dt*Dyy/hy^2 (main.cu:149-150)
+  float trx, try_;    // tc*rx, tc*ry
   float dt, tc, tcdt; // tcdt = tc*dt
   float rs;           // gate diffusion scale: rscale when gateDiff, else 0
-  // 2V reference reaction (reactionDiffusion.cu:131,141)
-  float mu, alpha, beta, gamma, delta, eps, theta;
-  // Fenton-Karma (reciprocal time constants)
-  float i_td, i_tr, i_2tsi, i_t0, i_tvp, i_tv1m, i_tv2m, i_twp, i_twm;
-  float u_c, u_v, u_csi, kk;
-  float hw;           // tanh-smoothed Heaviside width (0 = sharp)
+  // 2V reference reaction (reactionDiffusion.cu:131,141), folded with tc*dt:
+  //   -tcdt*I_sum = u*(a2_A*(1-u)*(u-alpha) - a2_B*v)
+  //   -tcdt*I_v   = a2_E*(u-gamma)*(beta-u) - a2_F*v - a2_G
+  float alpha, beta, gamma;
+  float a2_A, a2_B, a2_E, a2_F, a2_G;
+  // Fenton-Karma, folded with tc*dt (tau's as reciprocals)
+  float u_c, u_v;
+  float fk_A1, fk_A2, fk_A3, fk_A4;  // tcdt/tau_d, tcdt/tau_r, tcdt/tau_0, tcdt/tau_si
+  float fk_b, fk_bu;                 // 2k*log2(e), 2k*log2(e)*u_csi
+  float fk_B1, fk_B2, fk_B3;         // tcdt/tau_vp, tcdt/tau_v1m, tcdt/tau_v2m
+  float fk_C1, fk_C2;                // tcdt/tau_wp, tcdt/tau_wm
+  float hw;                          // tanh-smoothed Heaviside width (gradient path)
 };
 
 // Fields of the state (u, v, w); unused entries are null.
@@ -38,15 +45,11 @@ void make_kparams(const yh_params& p, const yh_derived& d, KParams* k);
 int nvars_of(int model);
 
 // ----------------------------------------------------------------- models
-// Reaction part of one explicit-Euler step.  Each functor returns the
-// increments -tc*dt*I for every variable; diffusion is added by the caller.
+// Reaction part of one explicit-Euler step: each functor returns the
+// increments -tc*dt*I of every variable; diffusion is added by the caller.
+// Selects are computed on both branches so the compiler emits v_cndmask, not
+// divergent branches.
 enum { M2V = 0, MFK = 1 };
-
-__device__ __forceinline__ float fast_tanh(float x) {
-  // tanh(x) = 1 - 2/(exp(2x)+1); saturates correctly for |x| large.
-  const float e = __expf(2.0f * x);
-  return 1.0f - 2.0f / (e + 1.0f);
-}
 
 template <int MODEL>
 struct Model;
@@ -58,10 +61,11 @@ struct Model<M2V> {
   __device__ __forceinline__ static void react(const KParams& k, float u, float v, float /*w*/,
                                                float& du, float& dv, float& /*dw*/) {
     // I_sum = -(mu u (1-u)(u-alpha) - u v);  I_v = -(eps (delta (u-gamma)(beta-u) - v - theta))
-    const float Is = -(k.mu * u * (1.0f - u) * (u - k.alpha) - u * v);
-    const float Iv = -(k.eps * (k.delta * (u - k.gamma) * (k.beta - u) - v - k.theta));
-    du = -k.tcdt * Is;
-    dv = -k.tcdt * Iv;
+    const float s2 = __builtin_fmaf(u, -k.a2_A, k.a2_A);  // A (1-u)
+    const float s3 = (u - k.alpha) * s2;
+    du = u * __builtin_fmaf(-k.a2_B, v, s3);
+    const float t3 = (u - k.gamma) * (k.beta - u);
+    dv = __builtin_fmaf(-k.a2_F, v, __builtin_fmaf(k.a2_E, t3, -k.a2_G));
   }
 };
 
@@ -71,18 +75,26 @@ struct Model<MFK> {
   static constexpr int ND = 1;  // only u diffuses
   __device__ __forceinline__ static void react(const KParams& k, float u, float v, float w,
                                                float& du, float& dv, float& dw) {
-    // Fenton & Karma 1998 with the sharp Heaviside STEP(a,b) = a >= b.
+    // Fenton & Karma 1998 with the sharp Heaviside STEP(a,b) = a >= b:
+    //   J_fi = -p v (1-u)(u-u_c)/tau_d,  J_so = (1-p) u/tau_0 + p/tau_r,
+    //   J_si = -w (1 + tanh(k(u-u_csi)))/(2 tau_si) = -w sigmoid(2k(u-u_csi))/tau_si
     const bool p = u >= k.u_c;
     const bool q = u >= k.u_v;
-    const float Jfi = p ? -v * (1.0f - u) * (u - k.u_c) * k.i_td : 0.0f;
-    const float Jso = p ? k.i_tr : u * k.i_t0;
-    const float th = fast_tanh(k.kk * (u - k.u_csi));
-    const float Jsi = -w * (1.0f + th) * k.i_2tsi;
-    const float Gv = p ? v * k.i_tvp : -(1.0f - v) * (q ? k.i_tv2m : k.i_tv1m);
-    const float Gw = p ? w * k.i_twp : -(1.0f - w) * k.i_twm;
-    du = -k.tcdt * (Jfi + Jso + Jsi);
-    dv = -k.tcdt * Gv;
-    dw = -k.tcdt * Gw;
+    const float s1 = u - k.u_c;
+    const float s2 = __builtin_fmaf(u, -k.fk_A1, k.fk_A1);           // A1 (1-u)
+    const float fi = __builtin_fmaf(s1 * s2, v, -k.fk_A2);           // -tcdt (J_fi + 1/tau_r)
+    const float so = -k.fk_A3 * u;                                   // -tcdt u/tau_0
+    const float r1 = p ? fi : so;
+    const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, k.fk_bu));
+    const float sg = __builtin_amdgcn_rcpf(ex + 1.0f);               // sigmoid(2k(u-u_csi))
+    du = r1 + (k.fk_A4 * w) * sg;
+    const float bq = q ? k.fk_B3 : k.fk_B2;
+    const float vo = __builtin_fmaf(-bq, v, bq);                     // (1-v) tcdt/tau_v-
+    const float vc = -k.fk_B1 * v;
+    dv = p ? vc : vo;
+    const float wo = __builtin_fmaf(-k.fk_C2, w, k.fk_C2);
+    const float wc = -k.fk_C1 * w;
+    dw = p ? wc : wo;
   }
 };
 
@@ -109,9 +121,10 @@ hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fi
                          const uint8_t* solid, hipStream_t s);
 hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields out,
                                const uint8_t* codes, hipStream_t s);
-hipError_t launch_tiled(int model, bool masked, int rows, int waves, int K, const KParams& k,
-                        CFields in, Fields out, const uint8_t* codes, hipStream_t s);
-bool tiled_supported(int rows, int waves);
+hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
+                        const KParams& k, CFields in, Fields out, const uint8_t* codes,
+                        hipStream_t s);
+bool tiled_supported(int rows, int waves, int cols);
 hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int tipOffsetY,
                        double Uth, const float* past, const float* present, double time,
                        yh_tip* tips, int* keys, int* count, int cap, hipStream_t s);

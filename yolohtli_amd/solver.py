@@ -167,8 +167,8 @@ class Plan:
         check(lib().yh_state_dev(self._h, C.byref(u), C.byref(v), C.byref(w)))
         return u.value, v.value, w.value
 
-    def set_kernel(self, variant=T.YH_KERNEL_AUTO, substeps=4, rows=8, waves=8):
-        cfg = T.YhKernelCfg(variant, substeps, rows, waves)
+    def set_kernel(self, variant=T.YH_KERNEL_AUTO, substeps=8, rows=8, waves=16, cols=1):
+        cfg = T.YhKernelCfg(variant, substeps, rows, waves, cols)
         check(lib().yh_set_kernel(self._h, C.byref(cfg)), "yh_set_kernel")
 
     def get_kernel(self):
@@ -197,6 +197,30 @@ class Plan:
 
     def clear_tips(self):
         check(lib().yh_clear_tips(self._h))
+
+    # ---- symmetry reduction (main.cu:894-953)
+    def sr_begin(self, tip0=(0.0, 0.0), stream=None):
+        check(lib().yh_sr_begin(self._h, float(tip0[0]), float(tip0[1]),
+                                _stream_handle(stream)), "yh_sr_begin")
+
+    def sr_step(self, nsteps, stream=None, history=False):
+        """nsteps SR steps; with history, returns (nsteps, 6) c and phi per step."""
+        out = np.empty((nsteps, 6)) if history else None
+        check(lib().yh_sr_step(self._h, int(nsteps), _stream_handle(stream),
+                               _ptr(out)), "yh_sr_step")
+        return out
+
+    def sr_set_graph(self, on):
+        check(lib().yh_sr_set_graph(self._h, int(bool(on))))
+
+    def sr_info(self):
+        info = T.YhSrInfo()
+        rc = lib().yh_sr_info_get(self._h, C.byref(info))
+        if rc not in (0, T.YH_ESINGULAR):
+            check(rc, "yh_sr_info_get")
+        return {"c": np.array(info.c[:]), "phi": np.array(info.phi[:]),
+                "I": np.array(info.I[:]), "cx": info.cx, "cy": info.cy,
+                "count": info.count, "status": info.status, "ntips": info.ntips}
 
     def set_option(self, option, value):
         check(lib().yh_set_option(self._h, int(option), int(value)), "yh_set_option")
