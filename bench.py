@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--kernel-launches", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only the dominant-kernel timing (for profiling); needs --cfg")
     return ap.parse_args()
 
 
@@ -67,6 +69,31 @@ def cpu_baseline(p, fields, mask, seconds):
             "kind": "port",
             "sample": f"{p.nx}x{p.ny} same model/mask/IC, {steps} steps, FP32, "
                       f"{cores} OpenMP threads, {el:.1f} s"}
+
+
+def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell):
+    """Average duration of the dominant kernel: `kernel_launches` back-to-back
+    launches of K fused steps each (no single-step tail, no tip kernel),
+    bracketed by two HIP events on the stream the kernel runs on."""
+    import torch
+    from yolohtli_amd import abi_types as T
+    pl.set_option(T.YH_OPT_KEEP_PREV, 0)
+    K = cfg[1] if cfg[0] == T.YH_KERNEL_TILED else 1
+    for _ in range(20):
+        pl.step(K, stream)
+    nl = a.kernel_launches
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    pl.step(K * nl, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_launch_ms = e0.elapsed_time(e1) / nl
+    pl.set_option(T.YH_OPT_KEEP_PREV, 1)
+    alg_bytes = n * bytes_per_cell  # per launch, independent of K
+    achieved = alg_bytes / (t_launch_ms / 1e3) / 1e9
+    return t_launch_ms, K, alg_bytes, achieved
 
 
 def main():
@@ -109,6 +136,13 @@ def main():
     pl.set_kernel(*cfg)
     pl.set_state(*fields)
     log(f"kernel cfg variant={cfg[0]} K={cfg[1]} rows={cfg[2]} waves={cfg[3]} cols={cfg[4]}")
+    if a.kernel_only:
+        t_ms, K, alg_bytes, achieved = kernel_timing(a, pl, cfg, stream, n, bytes_per_cell)
+        print(json.dumps({"kernel_launch_us": t_ms * 1e3, "K": K,
+                          "alg_bytes_per_launch": alg_bytes, "achieved_GBps": achieved,
+                          "gcell_steps_per_s_in_launch": n * K / (t_ms / 1e3) / 1e9}),
+              flush=True)
+        return
 
     def run(nsteps):
         done = 0
@@ -140,24 +174,7 @@ def main():
     tips = pl.read_tips()
     value = n * a.steps / (ms / 1e3) / 1e9
 
-    # ---- dominant kernel alone: one launch = K fused steps, events per launch
-    pl.set_option(T.YH_OPT_KEEP_PREV, 0)
-    K = cfg[1] if cfg[0] == T.YH_KERNEL_TILED else 1
-    for _ in range(20):
-        pl.step(K, stream)
-    nl = a.kernel_launches
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * nl)]
-    torch.cuda.synchronize()
-    for q in range(nl):
-        evs[2 * q].record(stream)
-        pl.step(K, stream)
-        evs[2 * q + 1].record(stream)
-    torch.cuda.synchronize()
-    durs = [evs[2 * q].elapsed_time(evs[2 * q + 1]) for q in range(nl)]
-    t_launch_ms = float(np.mean(durs))
-    alg_bytes = n * bytes_per_cell  # per launch, independent of K
-    achieved = alg_bytes / (t_launch_ms / 1e3) / 1e9
-    pl.set_option(T.YH_OPT_KEEP_PREV, 1)
+    t_launch_ms, K, alg_bytes, achieved = kernel_timing(a, pl, cfg, stream, n, bytes_per_cell)
     log(f"kernel: {t_launch_ms * 1e3:.2f} us per launch of K={K} "
         f"({n * K / (t_launch_ms / 1e3) / 1e9:.1f} Gcell-steps/s inside one launch)")
 

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(kSliceBX* kSliceBY) void k_sr_slices(
 
 // ----------------------------------------------------------------- solve
 __global__ __launch_bounds__(256) void k_sr_solve(SrGeom g, SrDev* st, const double* partials,
-                                                  int nparts, int pre, yh_tip* step_tips,
+                                                  int nparts, int pre, int record,
+                                                  yh_tip* step_tips,
                                                   int* step_keys, int* step_count,
                                                   yh_tip* log_tips, int* log_keys,
                                                   int* log_count, int logcap, double* hist,
@@ -205,8 +206,9 @@ __global__ __launch_bounds__(256) void k_sr_solve(SrGeom g, SrDev* st, const dou
       I[q] = red[0][q] * (g.hx * g.hy);
       st->I[q] = I[q];
     }
-    if (!pre && hist && st->count < histcap) {
-      // clist/philist push (main.cu:902-903): c and phi before this step's solve
+    if (record && hist && st->count < histcap) {
+      // clist/philist push (main.cu:902-903): c and phi before this step's
+      // first solve (at count 0 that is the preliminary solve, main.cu:910-914)
       double* h = hist + (size_t)st->count * 6;
       for (int q = 0; q < 3; ++q) {
         h[q] = st->c[q];
@@ -444,6 +446,7 @@ static int sr_enqueue(yh_plan* pl, bool first, hipStream_t s) {
   dim3 sb(kSliceBX, kSliceBY), sg((g.nx + kSliceBX - 1) / kSliceBX, (g.ny + kSliceBY - 1) / kSliceBY);
   for (int pass = first ? 0 : 1; pass < 2; ++pass) {
     const int pre = pass == 0;
+    const int record = pass == (first ? 0 : 1);
     // 4 slices + inner products
     if (pl->p.model == M2V) {
       if (masked) hipLaunchKernelGGL((k_sr_slices<M2V, true>), sg, sb, 0, s, pl->k, g, X, pl->codes_d, sr->st, sr->partials);
@@ -455,7 +458,8 @@ static int sr_enqueue(yh_plan* pl, bool first, hipStream_t s) {
     YH_HIP(hipGetLastError());
     // 5 solve
     hipLaunchKernelGGL(k_sr_solve, dim3(1), dim3(256), 0, s, g, sr->st, sr->partials, sr->nparts,
-                       pre, sr->step_tips, sr->step_keys, sr->step_count, pl->tips_d, pl->keys_d,
+                       pre, record, sr->step_tips, sr->step_keys, sr->step_count, pl->tips_d,
+                       pl->keys_d,
                        pl->tipcount_d, pl->tipcap, sr->hist, sr->histcap);
     YH_HIP(hipGetLastError());
   }
