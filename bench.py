@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--kernel-launches", type=int, default=300)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-tissues", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=5000)
+    ap.add_argument("--batch-each-steps", type=int, default=1000,
+                    help="steps of the per-tissue-launch comparison")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant-kernel timing (for profiling); needs --cfg")
     return ap.parse_args()
@@ -94,6 +99,56 @@ def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell):
     alg_bytes = n * bytes_per_cell  # per launch, independent of K
     achieved = alg_bytes / (t_launch_ms / 1e3) / 1e9
     return t_launch_ms, K, alg_bytes, achieved
+
+
+def batched(a):
+    """Batched configuration: --batch-tissues tissues of uneven size and
+    parameters, --batch-steps steps in one captured graph of work-list
+    launches, against per-tissue launches of the same kernel and k."""
+    import torch
+    from yolohtli_amd.batch import Batch, batched_workload
+    params, masks, fields = batched_workload(ntissues=a.batch_tissues)
+    b = Batch(params)
+    for t in range(len(params)):
+        b.set_mask(t, masks[t])
+        b.set_state(t, *fields[t])
+    del masks, fields
+    _, cells = b.size()
+    stream = torch.cuda.current_stream()
+    cfg, ms = b.autotune(steps=32, stream=stream)
+    cfg, pers = b.get_kernel()
+    log(f"batch autotune: K={cfg[1]} rows={cfg[2]} waves={cfg[3]} cols={cfg[4]} "
+        f"persistent={pers}: {ms:.3f} ms/step")
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    b.set_graph(True)
+    b.step(a.batch_steps, stream)  # captures the graph, warms up
+    torch.cuda.synchronize()
+    e0.record(stream)
+    b.step(a.batch_steps, stream)  # one graph launch
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_b = e0.elapsed_time(e1)
+    rate_b = cells * a.batch_steps / (ms_b / 1e3) / 1e9
+    log(f"batched: {a.batch_steps} steps of {cells} cells in {ms_b:.1f} ms = {rate_b:.1f} "
+        f"Gcell-steps/s")
+    n_each = a.batch_each_steps
+    b.step_each(cfg[1], stream)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    b.step_each(n_each, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_e = e0.elapsed_time(e1)
+    rate_e = cells * n_each / (ms_e / 1e3) / 1e9
+    log(f"per-tissue launches: {n_each} steps in {ms_e:.1f} ms = {rate_e:.1f} Gcell-steps/s")
+    b.close()
+    return {"tissues": len(params), "cells": int(cells), "steps": a.batch_steps,
+            "gcell_steps_per_s": round(rate_b, 3), "ms": round(ms_b, 3),
+            "per_tissue_launches_gcell_steps_per_s": round(rate_e, 3),
+            "per_tissue_steps": n_each, "speedup_vs_per_tissue": round(rate_b / rate_e, 3),
+            "kernel": {"K": cfg[1], "rows": cfg[2], "waves": cfg[3], "cols": cfg[4],
+                       "persistent": pers, "graph": True}}
 
 
 def main():
@@ -186,6 +241,12 @@ def main():
     log(f"kernel: {t_launch_ms * 1e3:.2f} us per launch of K={K} "
         f"({n * K / (t_launch_ms / 1e3) / 1e9:.1f} Gcell-steps/s inside one launch)")
 
+    pl.close()
+    batch = None
+    if not a.no_batch:
+        log("batched configuration ...")
+        batch = batched(a)
+
     cpu = None
     if not a.no_cpu:
         log("CPU baseline ...")
@@ -227,6 +288,7 @@ def main():
             "equiv_GBps_per_step": round(bytes_per_cell * n * K / (t_launch_ms / 1e3) / 1e9, 1),
         },
         "cpu_baseline": cpu,
+        "batched": batch,
         "tips_logged": int(len(tips)),
     }
     print(json.dumps(out), flush=True)

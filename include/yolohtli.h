@@ -213,6 +213,36 @@ int yh_sr_info_get(yh_plan* plan, yh_sr_info* out);
  * result is not finite. */
 int yh_solve_matrix(const double phi[3], const double I[12], double c[3]);
 
+/* --------------------------------------------------------------- batches */
+/* Many tissues advanced together (no reference counterpart: the reference
+ * steps one tissue, main.cu:876-885).  All tissues share model and mask mode;
+ * nx, ny, D and the ionic constants may differ.  One launch advances every
+ * tissue by k steps from a (tissue, tile) work list; yh_batch_step replays a
+ * captured hipGraph of the launches when graphs are on (default). */
+typedef struct yh_batch yh_batch;
+int yh_batch_create(const yh_params* per_tissue, int ntissues, yh_batch** out);
+void yh_batch_destroy(yh_batch* batch);
+int yh_batch_size(const yh_batch* batch, int* ntissues, int64_t* total_cells);
+int yh_batch_set_mask(yh_batch* batch, int tissue, const uint8_t* solid);
+int yh_batch_set_state(yh_batch* batch, int tissue, const float* u, const float* v,
+                       const float* w);
+int yh_batch_get_state(yh_batch* batch, int tissue, float* u, float* v, float* w);
+/* cfg: YH_KERNEL_TILED shape and substeps; persistent = 1 runs a grid of the
+ * resident workgroup count that loops over the work list, 0 one workgroup per
+ * item. */
+int yh_batch_set_kernel(yh_batch* batch, const yh_kernel_cfg* cfg, int persistent);
+int yh_batch_get_kernel(const yh_batch* batch, yh_kernel_cfg* cfg, int* persistent);
+int yh_batch_set_graph(yh_batch* batch, int on);
+int yh_batch_step(yh_batch* batch, int nsteps, void* stream);
+/* The same steps as per-tissue launches of the single-tissue kernel, one per
+ * tissue per k-step chunk (the comparison the batched launch is measured
+ * against). */
+int yh_batch_step_each(yh_batch* batch, int nsteps, void* stream);
+/* Times tile shapes x substep depths x {persistent, one-per-item} on the
+ * batch's own state (restored afterwards) and installs the fastest. */
+int yh_batch_autotune(yh_batch* batch, int steps, void* stream, yh_kernel_cfg* best,
+                      float* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,18 @@ PROTOTYPES = {
     "yh_sr_set_graph": (I, [P, I]),
     "yh_sr_info_get": (I, [P, C.POINTER(YhSrInfo)]),
     "yh_solve_matrix": (I, [P, P, P]),
+    "yh_batch_create": (I, [PARAMS_P, I, C.POINTER(P)]),
+    "yh_batch_destroy": (None, [P]),
+    "yh_batch_size": (I, [P, C.POINTER(I), C.POINTER(C.c_int64)]),
+    "yh_batch_set_mask": (I, [P, I, P]),
+    "yh_batch_set_state": (I, [P, I, P, P, P]),
+    "yh_batch_get_state": (I, [P, I, P, P, P]),
+    "yh_batch_set_kernel": (I, [P, C.POINTER(YhKernelCfg), I]),
+    "yh_batch_get_kernel": (I, [P, C.POINTER(YhKernelCfg), C.POINTER(I)]),
+    "yh_batch_set_graph": (I, [P, I]),
+    "yh_batch_step": (I, [P, I, P]),
+    "yh_batch_step_each": (I, [P, I, P]),
+    "yh_batch_autotune": (I, [P, I, P, C.POINTER(YhKernelCfg), C.POINTER(C.c_float)]),
 }
 
 

@@ -1,0 +1,200 @@
+// tile.h -- one temporally blocked tile of the plain stepper (device code),
+// shared by the single-tissue kernel (stepper.hip) and the batched work-list
+// kernel (batch.hip).
+#pragma once
+
+#include "yh_internal.h"
+
+namespace yh {
+
+__device__ __forceinline__ float lane_from(float v, int addr) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
+// Workgroup -> tile with consecutive tiles on the same XCD: workgroups are
+// dispatched round-robin over the 8 XCDs, so XCD x receives a contiguous band
+// of tiles and neighbouring halo rows hit the same L2.
+__device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
+  const int q = ntiles >> 3, rem = ntiles & 7;
+  const int x = bid & 7, idx = bid >> 3;
+  return x * q + (x < rem ? x : rem) + idx;
+}
+
+// Region of one workgroup: (64*CW) columns x (NW*R) rows; lane L owns columns
+// CW*L .. CW*L+CW-1, wave w owns rows w*R .. w*R+R-1.  After K substeps the
+// inner (64*CW-2K) x (NW*R-2K) cells are exact; the outer ring of width K is
+// recomputed by the neighbouring workgroup.  In substep s only rows that stay
+// exact after it, s+1 <= row < NW*R-s-1, are computed (the test is uniform
+// over a wave).  Rows whose cells all have the regular stencil (c = 1,1,1,1)
+// take the unmasked formula; the others decode per-cell coefficient bytes.
+template <int MODEL, int NW, int CW>
+struct TileLds {
+  // [buffer][first/last row][wave][var][column]
+  float xch[2][2][NW][Model<MODEL>::ND][64 * CW];
+};
+
+template <int MODEL, bool MASKED, int R, int NW, int CW>
+__device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Fields out,
+                                             const uint8_t* __restrict__ codes, int K, int tx,
+                                             int ty, TileLds<MODEL, NW, CW>& lds) {
+  using M = Model<MODEL>;
+  constexpr int NV = M::NV, ND = M::ND;
+  constexpr int WR = 64 * CW;
+  constexpr int H = NW * R;
+  auto& xch = lds.xch;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wout = WR - 2 * K;
+  const int hout = H - 2 * K;
+  const int nx = k.nx, ny = k.ny;
+  const int gx0 = tx * wout - K + lane * CW;
+  const int gy0 = ty * hout - K + wave * R;
+
+  int sx[CW];
+  bool flip_x[CW];
+#pragma unroll
+  for (int c = 0; c < CW; ++c) {
+    sx[c] = reflect_clamp(gx0 + c, nx);
+    flip_x[c] = (gx0 + c < 0) || (gx0 + c >= nx);
+  }
+
+  float X[NV][R][CW];
+  uint32_t cd[R][CW];   // coefficient bytes cW | cE<<8 | cN<<16 | cS<<24 (0 for holes)
+  uint32_t hole[CW];    // bit r: cell (r, c) is a hole
+  uint32_t irregular = 0;  // bit r: some cell of row r in this wave is not regular
+#pragma unroll
+  for (int c = 0; c < CW; ++c) hole[c] = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int gy = gy0 + r;
+    const int sy = reflect_clamp(gy, ny);
+    const bool flip_y = (gy < 0) || (gy >= ny);
+    bool odd = false;
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+      const size_t idx = (size_t)sy * nx + sx[c];
+#pragma unroll
+      for (int a = 0; a < NV; ++a) X[a][r][c] = in.f[a][idx];
+      if (MASKED) {
+        const uint32_t b = codes[idx];
+        const bool h = b == kHole;
+        const uint32_t q = h ? 0u : b;
+        uint32_t w2 = q & 3u, e2 = (q >> 2) & 3u, n2 = (q >> 4) & 3u, s2 = (q >> 6) & 3u;
+        if (flip_x[c]) { const uint32_t t = w2; w2 = e2; e2 = t; }
+        if (flip_y) { const uint32_t t = n2; n2 = s2; s2 = t; }
+        cd[r][c] = w2 | (e2 << 8) | (n2 << 16) | (s2 << 24);
+        hole[c] |= (h ? 1u : 0u) << r;
+        odd |= cd[r][c] != 0x01010101u;
+        if (h) {
+#pragma unroll
+          for (int a = 0; a < NV; ++a) X[a][r][c] = 0.0f;  // hole cells stay 0 (:521-522)
+        }
+      }
+    }
+    if (MASKED && __any(odd)) irregular |= 1u << r;
+  }
+
+  const int addr_w = ((lane + 63) & 63) << 2;
+  const int addr_e = ((lane + 1) & 63) << 2;
+
+  for (int s = 0; s < K; ++s) {
+    const int b = s & 1;
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        xch[b][0][wave][a][lane * CW + c] = X[a][0][c];
+        xch[b][1][wave][a][lane * CW + c] = X[a][R - 1][c];
+      }
+    __syncthreads();
+    float south[ND][CW], north[ND][CW];
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        south[a][c] = wave > 0 ? xch[b][1][wave - 1][a][lane * CW + c] : X[a][0][c];
+        north[a][c] = wave < NW - 1 ? xch[b][0][wave + 1][a][lane * CW + c] : X[a][R - 1][c];
+      }
+    float prev[ND][CW];
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < CW; ++c) prev[a][c] = south[a][c];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int lr = wave * R + r;
+      const bool active = (lr >= s + 1) && (lr < H - s - 1);
+      float old[ND][CW];
+#pragma unroll
+      for (int a = 0; a < ND; ++a)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) old[a][c] = X[a][r][c];
+      if (active) {
+        float lap[ND][CW];
+#pragma unroll
+        for (int a = 0; a < ND; ++a) {
+          const float wl = lane_from(old[a][CW - 1], addr_w);  // west of column 0
+          const float er = lane_from(old[a][0], addr_e);       // east of column CW-1
+#pragma unroll
+          for (int c = 0; c < CW; ++c) {
+            const float ctr = old[a][c];
+            const float wv = c == 0 ? wl : old[a][c - 1];
+            const float ev = c == CW - 1 ? er : old[a][c + 1];
+            const float sv = prev[a][c];
+            const float nv = (r < R - 1) ? X[a][r + 1][c] : north[a][c];
+            if (MASKED && ((irregular >> r) & 1u)) {
+              const uint32_t q = cd[r][c];
+              const float cW = (float)(q & 0xFFu), cE = (float)((q >> 8) & 0xFFu);
+              const float cN = (float)((q >> 16) & 0xFFu), cS = (float)(q >> 24);
+              const float lx = __builtin_fmaf(cE, ev - ctr, cW * (wv - ctr));
+              const float ly = __builtin_fmaf(cS, sv - ctr, cN * (nv - ctr));
+              lap[a][c] = __builtin_fmaf(k.try_, ly, k.trx * lx);
+            } else {
+              const float lx = __builtin_fmaf(-2.0f, ctr, wv + ev);
+              const float ly = __builtin_fmaf(-2.0f, ctr, nv + sv);
+              lap[a][c] = __builtin_fmaf(k.try_, ly, k.trx * lx);
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+          float rr[3] = {0.f, 0.f, 0.f};
+          M::react(k, X[0][r][c], X[1][r][c], NV > 2 ? X[NV - 1][r][c] : 0.0f, rr[0], rr[1], rr[2]);
+          X[0][r][c] += lap[0][c] + rr[0];
+          if (MODEL == M2V) {
+            X[1][r][c] += __builtin_fmaf(k.rs, lap[ND - 1][c], rr[1]);
+          } else {
+            X[1][r][c] += rr[1];
+            X[NV - 1][r][c] += rr[2];
+          }
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < ND; ++a)
+#pragma unroll
+        for (int c = 0; c < CW; ++c) prev[a][c] = old[a][c];
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int lr = wave * R + r;
+    const int gy = gy0 + r;
+    if (lr >= K && lr < H - K && gy < ny) {
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const int lc = lane * CW + c;
+        const int gx = gx0 + c;
+        if (lc >= K && lc < WR - K && gx < nx) {
+          const size_t idx = (size_t)gy * nx + gx;
+          const bool h = MASKED && ((hole[c] >> r) & 1u);
+#pragma unroll
+          for (int a = 0; a < NV; ++a) out.f[a][idx] = h ? 0.0f : X[a][r][c];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace yh
