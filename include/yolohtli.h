@@ -243,6 +243,29 @@ int yh_batch_step_each(yh_batch* batch, int nsteps, void* stream);
 int yh_batch_autotune(yh_batch* batch, int steps, void* stream, yh_kernel_cfg* best,
                       float* best_ms);
 
+/* --------------------------------------------------------------- adjoint */
+/* Gradient of loss = sum_{n % rec == 0} sum_s (u^n(sensor_s) - target)^2
+ * through nsteps of the FK stepper with a spatial diffusion field D and the
+ * tanh-smoothed Heaviside (params.heaviside_width > 0, model FK).  No
+ * reference counterpart.  Checkpoints every ckpt_every steps; the reverse
+ * sweep recomputes each segment.  Sensors are distinct cell indices. */
+typedef struct yh_adjoint yh_adjoint;
+int yh_adjoint_create(const yh_params* p, int nsteps, int rec_every, int ckpt_every,
+                      const int* sensors, int nsensors, yh_adjoint** out);
+void yh_adjoint_destroy(yh_adjoint* adj);
+int yh_adjoint_set_mask(yh_adjoint* adj, const uint8_t* solid);
+int yh_adjoint_set_initial(yh_adjoint* adj, const float* u, const float* v, const float* w);
+/* targets: (nsteps/rec_every) x nsensors, row = recorded step. */
+int yh_adjoint_set_targets(yh_adjoint* adj, const float* targets);
+/* d_D: device float32 field (nx*ny); it must stay valid until the matching
+ * yh_adjoint_backward.  tau6 = (tau_d, tau_r, tau_si, tau_0, tau_vp, tau_wp).
+ * loss is 0 until targets are set; traces (host, may be NULL) receives u at
+ * the sensors, one row per recorded step. */
+int yh_adjoint_forward(yh_adjoint* adj, const float* d_D, const double tau6[6], void* stream,
+                       double* loss, float* traces);
+/* d_gD: device float64 (nx*ny) receives dL/dD; gtau receives dL/dtau6. */
+int yh_adjoint_backward(yh_adjoint* adj, void* stream, double* d_gD, double gtau[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,8 @@ HIPFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
-SOURCES = ["api.cpp", "plan.cpp", "autotune.cpp", "stepper.hip", "sr.hip", "batch.hip"]
+SOURCES = ["api.cpp", "plan.cpp", "autotune.cpp", "stepper.hip", "sr.hip", "batch.hip",
+           "adjoint.hip"]
 
 
 def _compile(src):
