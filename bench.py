@@ -48,6 +48,12 @@ def parse():
     ap.add_argument("--batch-each-steps", type=int, default=1000,
                     help="steps of the per-tissue-launch comparison")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--adj-n", type=int, default=256)
+    ap.add_argument("--adj-steps", type=int, default=2000)
+    ap.add_argument("--adj-fd", type=int, default=1, help="FP64 FD check of tau gradients")
+    ap.add_argument("--adj-fd-taus", type=lambda s: [int(x) for x in s.split(",")],
+                    default=[0, 4])
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant-kernel timing (for profiling); needs --cfg")
     return ap.parse_args()
@@ -151,6 +157,60 @@ def batched(a):
                        "persistent": pers, "graph": True}}
 
 
+def adjoint(a):
+    """Gradient configuration: 256^2 FK with tanh-smoothed Heaviside (width
+    0.01), 2000 steps, 64 sensors recorded every step, checkpoint every 100;
+    targets from the hidden D field; gradient w.r.t. D (256^2) and 6 taus at
+    the uniform D0.  Reports the time per gradient and the relative error of
+    the tau-gradient against FP64 central differences of the oracle's loss."""
+    import torch
+    import oracle.oracle as O
+    from yolohtli_amd.adjoint import SensorProblem, TAU_NAMES, adjoint_workload
+    from yolohtli_amd import abi_types as T
+    n, nsteps = a.adj_n, a.adj_steps
+    p, x0, sensors, Dh, D0 = adjoint_workload(n=n, nsteps=nsteps)
+    pr = SensorProblem(p, nsteps, sensors, x0, rec_every=1, ckpt_every=100)
+    _, targets = pr.forward(torch.from_numpy(Dh.reshape(-1)).cuda())
+    pr.set_targets(targets)
+    D = torch.full((n * n,), D0, dtype=torch.float32, device="cuda")
+    L, _ = pr.forward(D)
+    pr.backward()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L, _ = pr.forward(D)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gD, gt = pr.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    log(f"adjoint: loss {L:.6e}; forward {1e3 * (t1 - t0):.1f} ms, backward "
+        f"{1e3 * (t2 - t1):.1f} ms")
+    out = {"n": n, "steps": nsteps, "sensors": int(len(sensors)), "checkpoint_every": 100,
+           "loss": L, "forward_ms": round(1e3 * (t1 - t0), 2),
+           "backward_ms": round(1e3 * (t2 - t1), 2),
+           "ms_per_gradient": round(1e3 * (t2 - t0), 2),
+           "grad_tau": [float(x) for x in gt], "grad_D_norm": float(gD.norm().item())}
+    if a.adj_fd:
+        X0 = [f.astype(np.float64) for f in x0]
+        Dd = np.full(n * n, D0, np.float64)
+        tg = targets.astype(np.float64)
+        errs = []
+        for k in a.adj_fd_taus:
+            name = TAU_NAMES[k]
+            tk = getattr(p.fk, name)
+            h = 1e-4 * tk
+            lp = O.fk_loss(T.copy_params(p, fk={name: tk + h}), Dd, X0, nsteps, 1, sensors, tg)
+            lm = O.fk_loss(T.copy_params(p, fk={name: tk - h}), Dd, X0, nsteps, 1, sensors, tg)
+            fd = (lp - lm) / (2 * h)
+            errs.append(abs(gt[k] - fd) / max(abs(fd), 1e-300))
+            log(f"adjoint FD check {name}: adjoint {gt[k]:.6e} FP64 central FD {fd:.6e}")
+        out["fd_check"] = {"taus": [TAU_NAMES[k] for k in a.adj_fd_taus],
+                           "max_rel_err": float(max(errs)), "h_rel": 1e-4,
+                           "reference": "FP64 oracle loss, central differences"}
+    pr.close()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -246,6 +306,10 @@ def main():
     if not a.no_batch:
         log("batched configuration ...")
         batch = batched(a)
+    adj = None
+    if not a.no_adjoint:
+        log("adjoint configuration ...")
+        adj = adjoint(a)
 
     cpu = None
     if not a.no_cpu:
@@ -289,6 +353,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batch,
+        "adjoint": adj,
         "tips_logged": int(len(tips)),
     }
     print(json.dumps(out), flush=True)

@@ -528,6 +528,7 @@ void orc_fk_dfield_step_d(const yh_params* p, const uint8_t* solid, const double
   orc_derive(p, &cf);
   const int nx = p->nx, ny = p->ny;
   const double rxh = cf.dt / (cf.hx * cf.hx), ryh = cf.dt / (cf.hy * cf.hy);
+#pragma omp parallel for schedule(static) if ((long)nx * ny >= 16384)
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
       const int c = i + nx * j;

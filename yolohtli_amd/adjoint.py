@@ -135,19 +135,27 @@ def sensor_loss(D, taus, problem):
 
 def adjoint_workload(n=256, nsteps=2000, nsensors=64, config=4, width=0.01, dx=0.025):
     """The gradient configuration: FK with a tanh-smoothed Heaviside, cross-field
-    start, hidden D = D0*(1 + sum of 4 Gaussian bumps), seeded sensor sites.
-    Returns (params, x0, sensors, D_hidden, D0)."""
+    start (a plane wave leaves the u = 1 strip at the west edge), hidden
+    D = D0*(1 + sum of 4 Gaussian bumps) and seeded sensor sites, both in the
+    west band the wave crosses during the run (x < 0.3 n), so the sensors see
+    the bumps.  Returns (params, x0, sensors, D_hidden, D0)."""
     rng = np.random.default_rng(seed(config, 0, 0) & 0xFFFFFFFF)
     p = T.grid_params(n, n, dx=dx, model=T.YH_MODEL_3V_FK, heaviside_width=width)
     from .solver import initGates
     x0 = initGates(n, n, u_cols=max(4, n // 16), rest=1.0, refr=0.0, nvars=3)
-    sensors = rng.choice(n * n, size=nsensors, replace=False).astype(np.int32)
+    cells = set()
+    while len(cells) < nsensors:
+        i = int(rng.integers(int(0.05 * n), int(0.3 * n)))
+        j = int(rng.integers(0, n))
+        cells.add(i + n * j)
+    sensors = np.array(sorted(cells), np.int32)
     D0 = 0.001
     jj, ii = np.mgrid[0:n, 0:n].astype(np.float64)
     bump = np.zeros((n, n))
     for _ in range(4):
-        xm, ym = rng.uniform(0.2, 0.8, size=2) * n
-        sig = rng.uniform(0.05, 0.15) * n
+        xm = rng.uniform(0.08, 0.25) * n
+        ym = rng.uniform(0.1, 0.9) * n
+        sig = rng.uniform(0.04, 0.08) * n
         amp = rng.uniform(-0.3, 0.3)
         bump += amp * np.exp(-((ii - xm) ** 2 + (jj - ym) ** 2) / (2 * sig ** 2))
     D_hidden = (D0 * (1.0 + bump)).astype(np.float32)
