@@ -175,6 +175,19 @@ int yh_check_finite(yh_plan* plan, void* stream);
 enum { YH_OPT_KEEP_PREV = 1 };
 int yh_set_option(yh_plan* plan, int option, int64_t value);
 
+/* Action-potential durations measured while stepping (sAPD_kernel,
+ * spaceAPD.cu:278-374, stimulus off; threshold apdTh = 0.15 at :288).  Every
+ * step updates, per cell, the up-crossing time `front`, the down-crossing
+ * time `back` (both linearly interpolated, :298-306) and, once both are set,
+ * the alternating durations apd1/apd2 = back - front.  The update is fused
+ * into the stepping kernels; yh_apd_enable zeroes the state.  The crossing
+ * arguments are passed in time order (old state, new state): display() passes
+ * them swapped after its pointer swap (main.cu:882,1035-1038), which makes
+ * its durations negative. */
+int yh_apd_enable(yh_plan* plan, int on, double threshold);
+/* Host copies of the per-cell fields; any pointer may be NULL. */
+int yh_apd_get(yh_plan* plan, float* apd1, float* apd2, float* front, float* back);
+
 /* Autotuner (no reference counterpart; the reference launches fixed 16x16
  * blocks, globalVariables.cuh:15-16): times the direct kernel and the tiled
  * kernel over tile shapes x substep depths on the plan's own state (restored

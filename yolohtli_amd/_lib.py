@@ -48,6 +48,8 @@ PROTOTYPES = {
     "yh_clear_tips": (I, [P]),
     "yh_check_finite": (I, [P, P]),
     "yh_set_option": (I, [P, I, C.c_int64]),
+    "yh_apd_enable": (I, [P, I, D]),
+    "yh_apd_get": (I, [P, P, P, P, P]),
     "yh_autotune": (I, [P, I, P, C.POINTER(YhKernelCfg), C.POINTER(C.c_float)]),
     "yh_sr_begin": (I, [P, C.c_float, C.c_float, P]),
     "yh_sr_step": (I, [P, I, P, P]),

@@ -30,6 +30,7 @@ std::vector<yh_kernel_cfg> candidates(const yh_plan* pl) {
       const int rows = s[0], waves = s[1], cols = s[2];
       if (2 * K >= 64 * cols || 2 * K >= rows * waves) continue;
       if (!yh::tiled_supported(rows, waves, cols)) continue;
+      if (pl->apd_on && !yh::tiled_apd_supported(rows, waves, cols)) continue;
       c.push_back({YH_KERNEL_TILED, K, rows, waves, cols});
     }
   }
@@ -67,13 +68,16 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
   const bool keep0 = pl->keep_prev, prev0 = pl->have_prev;
   const yh_kernel_cfg cfg0 = pl->cfg;
   pl->keep_prev = false;
+  const bool apd0 = pl->apd_on;  // the timed steps must not touch the APD state
+  std::vector<yh_kernel_cfg> cands = candidates(pl);
+  pl->apd_on = false;
   hipEvent_t e0, e1;
   YH_HIP(hipEventCreate(&e0));
   YH_HIP(hipEventCreate(&e1));
   yh_kernel_cfg bestc = cfg0;
   float bestt = 1e30f;
   int rc = YH_OK;
-  for (const yh_kernel_cfg& c : candidates(pl)) {
+  for (const yh_kernel_cfg& c : cands) {
     pl->cfg = c;
     const int K = c.variant == YH_KERNEL_TILED ? c.substeps : 1;
     const int n = std::max(K, (steps / K) * K);
@@ -103,6 +107,7 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
   for (int a = 0; a < pl->nv; ++a) (void)hipFree(save[a]);
   pl->count = count0;
   pl->keep_prev = keep0;
+  pl->apd_on = apd0;
   (void)prev0;
   pl->have_prev = false;  // buf[cur^1] holds a candidate's output, not the previous step
   if (rc) {

@@ -16,13 +16,18 @@ yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
   if (c.variant == YH_KERNEL_AUTO) {
     c.variant = YH_KERNEL_TILED;
     c.substeps = 8;
-    c.rows = 8;
+    c.rows = 4;
     c.waves = 16;
     c.cols = 1;
   }
   if (c.variant == YH_KERNEL_TILED) {
     const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
     c.substeps = std::max(1, std::min(c.substeps, kmax));
+    if (pl->apd_on && !tiled_apd_supported(c.rows, c.waves, c.cols)) {
+      c.rows = 4;  // the fused APD update is built for these shapes only
+      c.waves = 16;
+      c.cols = 1;
+    }
   }
   return c;
 }
@@ -38,6 +43,9 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
     *ci = CFields{{pl->buf[b][0], pl->buf[b][1], pl->buf[b][2]}};
     *co = Fields{{pl->buf[b ^ 1][0], pl->buf[b ^ 1][1], pl->buf[b ^ 1][2]}};
   };
+  ApdState apd = pl->apd;
+  const ApdState* ap = pl->apd_on ? &apd : nullptr;
+  int64_t done = pl->count;
   const int tail = pl->keep_prev ? 1 : 0;
   if (c.variant == YH_KERNEL_TILED) {
     while (remaining > tail) {
@@ -45,19 +53,24 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
       CFields ci;
       Fields co;
       fields(pl->cur, &ci, &co);
+      apd.count0 = (int)done;
       YH_HIP(launch_tiled(pl->p.model, masked, c.rows, c.waves, c.cols, K, pl->k, ci, co,
-                          masked ? pl->codes_d : nullptr, s));
+                          masked ? pl->codes_d : nullptr, s, ap));
       pl->cur ^= 1;
       remaining -= K;
+      done += K;
     }
   }
   while (remaining > 0) {
     CFields ci;
     Fields co;
     fields(pl->cur, &ci, &co);
-    YH_HIP(launch_direct_codes(pl->p.model, pl->k, ci, co, masked ? pl->codes_d : nullptr, s));
+    apd.count0 = (int)done;
+    YH_HIP(launch_direct_codes(pl->p.model, pl->k, ci, co, masked ? pl->codes_d : nullptr, s,
+                               ap));
     pl->cur ^= 1;
     remaining -= 1;
+    done += 1;
   }
   pl->have_prev = c.variant != YH_KERNEL_TILED || pl->keep_prev || c.substeps == 1;
   pl->count += nsteps;
@@ -75,6 +88,9 @@ static void plan_free(yh_plan* pl) {
   if (pl->keys_d) (void)hipFree(pl->keys_d);
   if (pl->tipcount_d) (void)hipFree(pl->tipcount_d);
   if (pl->flag_d) (void)hipFree(pl->flag_d);
+  void* apd[] = {pl->apd.front, pl->apd.back, pl->apd.apd1, pl->apd.apd2, pl->apd.first};
+  for (void* q : apd)
+    if (q) (void)hipFree(q);
   if (pl->sr) sr_free(pl->sr);
   delete pl;
 }
@@ -253,6 +269,43 @@ int yh_set_option(yh_plan* pl, int option, int64_t value) {
     default:
       return YH_EINVAL;
   }
+}
+
+int yh_apd_enable(yh_plan* pl, int on, double threshold) {
+  if (!pl) return YH_EINVAL;
+  if (!on) {
+    pl->apd_on = false;
+    return YH_OK;
+  }
+  if (!pl->apd.front) {
+    YH_HIP(hipMalloc(&pl->apd.front, pl->n * sizeof(float)));
+    YH_HIP(hipMalloc(&pl->apd.back, pl->n * sizeof(float)));
+    YH_HIP(hipMalloc(&pl->apd.apd1, pl->n * sizeof(float)));
+    YH_HIP(hipMalloc(&pl->apd.apd2, pl->n * sizeof(float)));
+    YH_HIP(hipMalloc(&pl->apd.first, pl->n));
+  }
+  YH_HIP(hipDeviceSynchronize());
+  YH_HIP(hipMemset(pl->apd.front, 0, pl->n * sizeof(float)));
+  YH_HIP(hipMemset(pl->apd.back, 0, pl->n * sizeof(float)));
+  YH_HIP(hipMemset(pl->apd.apd1, 0, pl->n * sizeof(float)));
+  YH_HIP(hipMemset(pl->apd.apd2, 0, pl->n * sizeof(float)));
+  YH_HIP(hipMemset(pl->apd.first, 0, pl->n));
+  pl->apd.th = (float)threshold;
+  pl->apd.dt = (float)pl->d.dt;
+  pl->apd_on = true;
+  return YH_OK;
+}
+
+int yh_apd_get(yh_plan* pl, float* apd1, float* apd2, float* front, float* back) {
+  if (!pl) return YH_EINVAL;
+  if (!pl->apd.front) return YH_ESTATE;
+  YH_HIP(hipDeviceSynchronize());
+  const size_t b = pl->n * sizeof(float);
+  if (apd1) YH_HIP(hipMemcpy(apd1, pl->apd.apd1, b, hipMemcpyDeviceToHost));
+  if (apd2) YH_HIP(hipMemcpy(apd2, pl->apd.apd2, b, hipMemcpyDeviceToHost));
+  if (front) YH_HIP(hipMemcpy(front, pl->apd.front, b, hipMemcpyDeviceToHost));
+  if (back) YH_HIP(hipMemcpy(back, pl->apd.back, b, hipMemcpyDeviceToHost));
+  return YH_OK;
 }
 
 int yh_check_finite(yh_plan* pl, void* stream) {

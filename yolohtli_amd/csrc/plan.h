@@ -28,6 +28,8 @@ struct yh_plan {
   yh_kernel_cfg cfg{YH_KERNEL_AUTO, 4, 8, 8, 1};
   int64_t count = 0;
   bool keep_prev = true;
+  bool apd_on = false;
+  yh::ApdState apd;  // device arrays (front, back, apd1, apd2, first) when apd_on
   yh_sr* sr = nullptr;
 };
 

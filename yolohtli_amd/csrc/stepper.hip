@@ -113,9 +113,10 @@ __global__ __launch_bounds__(256) void k_direct(const KParams k, CFields in, Fie
   }
 }
 
-template <int MODEL>
+template <int MODEL, bool APD>
 __global__ __launch_bounds__(256) void k_direct_codes(const KParams k, CFields in, Fields out,
-                                                      const uint8_t* __restrict__ codes) {
+                                                      const uint8_t* __restrict__ codes,
+                                                      const ApdState apd) {
   using M = Model<MODEL>;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -139,6 +140,16 @@ __global__ __launch_bounds__(256) void k_direct_codes(const KParams k, CFields i
   cell_update<MODEL>(k, in.f, c, W, E, N, S, cW, cE, cN, cS, sc, xo, inc);
 #pragma unroll
   for (int a = 0; a < M::NV; ++a) out.f[a][c] = xo[a];
+  if (APD && sc) {
+    float fr = apd.front[c], bk = apd.back[c], d1 = apd.apd1[c], d2 = apd.apd2[c];
+    uint32_t fs = apd.first[c];
+    apd_cell(in.f[0][c], xo[0], apd.count0 + 1, apd.dt, apd.th, fr, bk, d1, d2, fs);
+    apd.front[c] = fr;
+    apd.back[c] = bk;
+    apd.apd1[c] = d1;
+    apd.apd2[c] = d2;
+    apd.first[c] = (uint8_t)fs;
+  }
 }
 
 hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fields vt,
@@ -155,22 +166,29 @@ hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fi
 }
 
 hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields out,
-                               const uint8_t* codes, hipStream_t s) {
+                               const uint8_t* codes, hipStream_t s, const ApdState* apd) {
   dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
-  if (model == M2V) hipLaunchKernelGGL((k_direct_codes<M2V>), g, b, 0, s, k, in, out, codes);
-  else hipLaunchKernelGGL((k_direct_codes<MFK>), g, b, 0, s, k, in, out, codes);
+  const ApdState a = apd ? *apd : ApdState();
+  if (model == M2V) {
+    if (apd) hipLaunchKernelGGL((k_direct_codes<M2V, true>), g, b, 0, s, k, in, out, codes, a);
+    else hipLaunchKernelGGL((k_direct_codes<M2V, false>), g, b, 0, s, k, in, out, codes, a);
+  } else {
+    if (apd) hipLaunchKernelGGL((k_direct_codes<MFK, true>), g, b, 0, s, k, in, out, codes, a);
+    else hipLaunchKernelGGL((k_direct_codes<MFK, false>), g, b, 0, s, k, in, out, codes, a);
+  }
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------------- tiled
-template <int MODEL, bool MASKED, int R, int NW, int CW>
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD>
 __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, Fields out,
                                                    const uint8_t* __restrict__ codes, int K,
-                                                   int tiles_x, int ntiles) {
+                                                   int tiles_x, int ntiles, const ApdState apd) {
   __shared__ TileLds<MODEL, NW, CW> lds;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int ty = tile / tiles_x;
-  tile_advance<MODEL, MASKED, R, NW, CW>(k, in, out, codes, K, tile - ty * tiles_x, ty, lds);
+  tile_advance<MODEL, MASKED, R, NW, CW, APD>(k, in, out, codes, K, tile - ty * tiles_x, ty, lds,
+                                              apd);
 }
 
 bool tiled_supported(int rows, int waves, int cols) {
@@ -181,25 +199,43 @@ bool tiled_supported(int rows, int waves, int cols) {
   return rw && (cols == 1 || (cols == 2 && rows * waves <= 64 && rows <= 8));
 }
 
-template <int MODEL, bool MASKED, int R, int NW, int CW>
+// Shapes built with the fused APD update (one column per lane, at most 8 rows).
+bool tiled_apd_supported(int rows, int waves, int cols) {
+  return cols == 1 && (rows == 4 || rows == 8) && (waves == 8 || waves == 16);
+}
+
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD>
 static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out,
-                                 const uint8_t* codes, hipStream_t s) {
+                                 const uint8_t* codes, hipStream_t s, const ApdState& apd) {
   const int wout = 64 * CW - 2 * K, hout = NW * R - 2 * K;
   if (wout <= 0 || hout <= 0) return hipErrorInvalidValue;
   const int tiles_x = (k.nx + wout - 1) / wout;
   const int tiles_y = (k.ny + hout - 1) / hout;
   const int ntiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW>), dim3(ntiles), dim3(64 * NW), 0, s, k,
-                     in, out, codes, K, tiles_x, ntiles);
+  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW, APD>), dim3(ntiles), dim3(64 * NW), 0, s,
+                     k, in, out, codes, K, tiles_x, ntiles, apd);
   return hipGetLastError();
 }
 
 template <int MODEL, bool MASKED>
 static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KParams& k,
-                                 CFields in, Fields out, const uint8_t* codes, hipStream_t s) {
+                                 CFields in, Fields out, const uint8_t* codes, hipStream_t s,
+                                 const ApdState* apd) {
+  const ApdState a = apd ? *apd : ApdState();
+  if (apd) {
+#define YH_TILE_APD(RR, WW)                    \
+  if (rows == RR && waves == WW && cols == 1) \
+    return launch_tiled_t<MODEL, MASKED, RR, WW, 1, true>(K, k, in, out, codes, s, a);
+    YH_TILE_APD(4, 8)
+    YH_TILE_APD(8, 8)
+    YH_TILE_APD(4, 16)
+    YH_TILE_APD(8, 16)
+#undef YH_TILE_APD
+    return hipErrorInvalidValue;
+  }
 #define YH_TILE_CASE(RR, WW, CC)                 \
   if (rows == RR && waves == WW && cols == CC) \
-    return launch_tiled_t<MODEL, MASKED, RR, WW, CC>(K, k, in, out, codes, s);
+    return launch_tiled_t<MODEL, MASKED, RR, WW, CC, false>(K, k, in, out, codes, s, a);
   YH_TILE_CASE(4, 4, 1)
   YH_TILE_CASE(8, 4, 1)
   YH_TILE_CASE(16, 4, 1)
@@ -219,13 +255,13 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
 
 hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
                         const KParams& k, CFields in, Fields out, const uint8_t* codes,
-                        hipStream_t s) {
+                        hipStream_t s, const ApdState* apd) {
   if (model == M2V) {
-    return masked ? launch_tiled_m<M2V, true>(rows, waves, cols, K, k, in, out, codes, s)
-                  : launch_tiled_m<M2V, false>(rows, waves, cols, K, k, in, out, codes, s);
+    return masked ? launch_tiled_m<M2V, true>(rows, waves, cols, K, k, in, out, codes, s, apd)
+                  : launch_tiled_m<M2V, false>(rows, waves, cols, K, k, in, out, codes, s, apd);
   }
-  return masked ? launch_tiled_m<MFK, true>(rows, waves, cols, K, k, in, out, codes, s)
-                : launch_tiled_m<MFK, false>(rows, waves, cols, K, k, in, out, codes, s);
+  return masked ? launch_tiled_m<MFK, true>(rows, waves, cols, K, k, in, out, codes, s, apd)
+                : launch_tiled_m<MFK, false>(rows, waves, cols, K, k, in, out, codes, s, apd);
 }
 
 // -------------------------------------------------------------------- tips

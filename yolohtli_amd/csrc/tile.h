@@ -33,10 +33,14 @@ struct TileLds {
   float xch[2][2][NW][Model<MODEL>::ND][64 * CW];
 };
 
-template <int MODEL, bool MASKED, int R, int NW, int CW>
+// With APD, every substep also updates the action-potential-duration state of
+// the cells this tile outputs (their value is exact at every substep), so
+// the measurement costs no extra launch or state traffic.
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false>
 __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Fields out,
                                              const uint8_t* __restrict__ codes, int K, int tx,
-                                             int ty, TileLds<MODEL, NW, CW>& lds) {
+                                             int ty, TileLds<MODEL, NW, CW>& lds,
+                                             const ApdState& apd = ApdState()) {
   using M = Model<MODEL>;
   constexpr int NV = M::NV, ND = M::ND;
   constexpr int WR = 64 * CW;
@@ -93,6 +97,26 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
       }
     }
     if (MASKED && __any(odd)) irregular |= 1u << r;
+  }
+
+  // APD state of this lane's cells (only output cells are written back)
+  float a_front[APD ? R : 1][CW], a_back[APD ? R : 1][CW];
+  float a_d1[APD ? R : 1][CW], a_d2[APD ? R : 1][CW];
+  uint32_t a_first[APD ? R : 1][CW];
+  if (APD) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const size_t row = (size_t)reflect_clamp(gy0 + r, ny) * nx;
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const size_t idx = row + sx[c];
+        a_front[APD ? r : 0][c] = apd.front[idx];
+        a_back[APD ? r : 0][c] = apd.back[idx];
+        a_d1[APD ? r : 0][c] = apd.apd1[idx];
+        a_d2[APD ? r : 0][c] = apd.apd2[idx];
+        a_first[APD ? r : 0][c] = apd.first[idx];
+      }
+    }
   }
 
   const int addr_w = ((lane + 63) & 63) << 2;
@@ -161,7 +185,13 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
         for (int c = 0; c < CW; ++c) {
           float rr[3] = {0.f, 0.f, 0.f};
           M::react(k, X[0][r][c], X[1][r][c], NV > 2 ? X[NV - 1][r][c] : 0.0f, rr[0], rr[1], rr[2]);
+          const float u_old = X[0][r][c];
           X[0][r][c] += lap[0][c] + rr[0];
+          if (APD && !(MASKED && ((hole[c] >> r) & 1u))) {
+            apd_cell(u_old, X[0][r][c], apd.count0 + s + 1, apd.dt, apd.th,
+                     a_front[APD ? r : 0][c], a_back[APD ? r : 0][c], a_d1[APD ? r : 0][c],
+                     a_d2[APD ? r : 0][c], a_first[APD ? r : 0][c]);
+          }
           if (MODEL == M2V) {
             X[1][r][c] += __builtin_fmaf(k.rs, lap[ND - 1][c], rr[1]);
           } else {
@@ -191,6 +221,13 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
           const bool h = MASKED && ((hole[c] >> r) & 1u);
 #pragma unroll
           for (int a = 0; a < NV; ++a) out.f[a][idx] = h ? 0.0f : X[a][r][c];
+          if (APD) {
+            apd.front[idx] = a_front[APD ? r : 0][c];
+            apd.back[idx] = a_back[APD ? r : 0][c];
+            apd.apd1[idx] = a_d1[APD ? r : 0][c];
+            apd.apd2[idx] = a_d2[APD ? r : 0][c];
+            apd.first[idx] = (uint8_t)a_first[APD ? r : 0][c];
+          }
         }
       }
     }

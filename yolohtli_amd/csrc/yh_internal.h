@@ -44,6 +44,40 @@ struct CFields {
 void make_kparams(const yh_params& p, const yh_derived& d, KParams* k);
 int nvars_of(int model);
 
+// Per-cell action-potential-duration state of sAPD_kernel (spaceAPD.cu:278-374):
+// threshold-crossing times front/back and the two alternating durations.
+struct ApdState {
+  float* front = nullptr;
+  float* back = nullptr;
+  float* apd1 = nullptr;
+  float* apd2 = nullptr;
+  uint8_t* first = nullptr;
+  float th = 0.15f;  // apdTh, spaceAPD.cu:288
+  float dt = 0.0f;
+  int count0 = 0;    // step count before the launch
+};
+
+// One sAPD update (spaceAPD.cu:328-358, stimulus off) for a cell going from
+// uo to un at step `count`.
+__device__ __forceinline__ void apd_cell(float uo, float un, int count, float dt, float th,
+                                         float& front, float& back, float& apd1, float& apd2,
+                                         uint32_t& first) {
+  if ((un > th) && (uo < th)) front = dt * ((float)count - (un - th) / (un - uo));
+  if ((un < th) && (uo > th)) back = dt * ((float)count - (un - th) / (un - uo));
+  if ((back > 0.0f) && (front > 0.0f) && (first == 0u)) {
+    apd1 = back - front;
+    front = 0.0f;
+    back = 0.0f;
+    first = 1u;
+  }
+  if ((back > 0.0f) && (front > 0.0f) && (first != 0u)) {
+    apd2 = back - front;
+    front = 0.0f;
+    back = 0.0f;
+    first = 0u;
+  }
+}
+
 // ----------------------------------------------------------------- models
 // Reaction part of one explicit-Euler step: each functor returns the
 // increments -tc*dt*I of every variable; diffusion is added by the caller.
@@ -120,11 +154,13 @@ hipError_t launch_codes(const uint8_t* solid, uint8_t* codes, int nx, int ny, hi
 hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fields vt,
                          const uint8_t* solid, hipStream_t s);
 hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields out,
-                               const uint8_t* codes, hipStream_t s);
+                               const uint8_t* codes, hipStream_t s,
+                               const ApdState* apd = nullptr);
 hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
                         const KParams& k, CFields in, Fields out, const uint8_t* codes,
-                        hipStream_t s);
+                        hipStream_t s, const ApdState* apd = nullptr);
 bool tiled_supported(int rows, int waves, int cols);
+bool tiled_apd_supported(int rows, int waves, int cols);
 hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int tipOffsetY,
                        double Uth, const float* past, const float* present, double time,
                        yh_tip* tips, int* keys, int* count, int cap, hipStream_t s);

@@ -222,6 +222,17 @@ class Plan:
                 "I": np.array(info.I[:]), "cx": info.cx, "cy": info.cy,
                 "count": info.count, "status": info.status, "ntips": info.ntips}
 
+    # ---- action-potential durations (spaceAPD.cu:278-374)
+    def apd_enable(self, threshold=0.15, on=True):
+        check(lib().yh_apd_enable(self._h, int(bool(on)), float(threshold)), "yh_apd_enable")
+
+    def apd_get(self):
+        """dict of per-cell apd1, apd2, front, back (float32)."""
+        out = {k: np.empty(self.n, np.float32) for k in ("apd1", "apd2", "front", "back")}
+        check(lib().yh_apd_get(self._h, _ptr(out["apd1"]), _ptr(out["apd2"]), _ptr(out["front"]),
+                               _ptr(out["back"])), "yh_apd_get")
+        return out
+
     def set_option(self, option, value):
         check(lib().yh_set_option(self._h, int(option), int(value)), "yh_set_option")
 
