@@ -666,3 +666,42 @@ double orc_fk_adjoint_d(const yh_params* p, const uint8_t* solid, const double* 
   free(traj);
   return L;
 }
+
+/* nsteps of the plain loop with the sAPD update after every step
+ * (spaceAPD.cu:328-358, stimulus off), crossings taken in time order
+ * (uold = state before the step, unew = after), threshold th. */
+void orc_run_apd_d(const yh_params* p, const uint8_t* solid, double* const st[3], int nsteps,
+                   long count0, double th, double* apd1, double* apd2, double* front,
+                   double* back, uint8_t* first) {
+  orc_coef cf;
+  orc_derive(p, &cf);
+  const size_t n = (size_t)p->nx * p->ny;
+  const int nv = p->model == YH_MODEL_3V_FK ? 3 : 2;
+  double* nw[3] = {0, 0, 0};
+  for (int a = 0; a < nv; ++a) nw[a] = (double*)malloc(n * sizeof(double));
+  for (int s = 0; s < nsteps; ++s) {
+    const double* cin[3] = {st[0], st[1], nv == 3 ? st[2] : 0};
+    orc_rd_step_d(p, solid, cin, nw, 0);
+    const long count = count0 + s + 1;
+    for (size_t k = 0; k < n; ++k) {
+      if (p->solidSwitch && !solid[k]) continue;
+      const double uo = st[0][k], un = nw[0][k];
+      if ((un > th) && (uo < th)) front[k] = cf.dt * (count - (un - th) / (un - uo));
+      if ((un < th) && (uo > th)) back[k] = cf.dt * (count - (un - th) / (un - uo));
+      if ((back[k] > 0.0) && (front[k] > 0.0) && (first[k] == 0)) {
+        apd1[k] = back[k] - front[k];
+        front[k] = 0.0;
+        back[k] = 0.0;
+        first[k] = 1;
+      }
+      if ((back[k] > 0.0) && (front[k] > 0.0) && first[k]) {
+        apd2[k] = back[k] - front[k];
+        front[k] = 0.0;
+        back[k] = 0.0;
+        first[k] = 0;
+      }
+    }
+    for (int a = 0; a < nv; ++a) memcpy(st[a], nw[a], n * sizeof(double));
+  }
+  for (int a = 0; a < nv; ++a) free(nw[a]);
+}

@@ -95,6 +95,12 @@ void orc_apd_update_d(int ncell, double dt, long count, const double* uold,
                       const double* unew, double* apd1, double* apd2, double* front,
                       double* back, uint8_t* first);
 
+/* orc_run_d with the sAPD update after every step, crossings in time order
+ * (old state, new state) and threshold th; apd arrays updated in place. */
+void orc_run_apd_d(const yh_params* p, const uint8_t* solid, double* const st[3], int nsteps,
+                   long count0, double th, double* apd1, double* apd2, double* front,
+                   double* back, uint8_t* first);
+
 /* ---------------------------------------------------------------- adjoint */
 /* FK forward with a spatial diffusion field D (cm^2/ms per cell), used by the
  * gradient path: u' = u + tc*(L_D u - dt*I), with

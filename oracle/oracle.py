@@ -60,6 +60,7 @@ def lib():
         h.orc_sr_step_d.argtypes = [PP, P, P, C.POINTER(OrcSrState)]
         h.orc_sr_step_d.restype = C.c_int
         h.orc_apd_update_d.argtypes = [C.c_int, C.c_double, C.c_long, P, P, P, P, P, P, P]
+        h.orc_run_apd_d.argtypes = [PP, P, P, C.c_int, C.c_long, C.c_double, P, P, P, P, P]
         h.orc_fk_dfield_step_d.argtypes = [PP, P, P, P, P]
         h.orc_fk_loss_d.argtypes = [PP, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
         h.orc_fk_loss_d.restype = C.c_double
@@ -238,3 +239,19 @@ def apd_update(dt, count, uold, unew, st):
     lib().orc_apd_update_d(n, float(dt), int(count), _p(np.ascontiguousarray(uold, np.float64)),
                            _p(np.ascontiguousarray(unew, np.float64)), _p(st["apd1"]),
                            _p(st["apd2"]), _p(st["front"]), _p(st["back"]), _p(st["first"]))
+
+
+def run_apd(p, fields, nsteps, solid=None, th=0.15, count0=0, state=None):
+    """nsteps with the APD update; returns (fields, apd state dict)."""
+    nv = nvars(p)
+    n = p.nx * p.ny
+    fs = [np.array(f, dtype=np.float64).reshape(-1) for f in fields[:nv]]
+    cst = (C.c_void_p * 3)(*([_p(f) for f in fs] + [None] * (3 - nv)))
+    st = state or {k: np.zeros(n) for k in ("apd1", "apd2", "front", "back")}
+    if "first" not in st:
+        st["first"] = np.zeros(n, np.uint8)
+    m = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+    lib().orc_run_apd_d(C.byref(p), _p(m), cst, int(nsteps), int(count0), float(th),
+                        _p(st["apd1"]), _p(st["apd2"]), _p(st["front"]), _p(st["back"]),
+                        _p(st["first"]))
+    return fs, st

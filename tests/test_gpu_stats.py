@@ -1,0 +1,92 @@
+"""Long-run statistics on the GPU against the FP64 oracle.
+
+FP32 and FP64 trajectories of an excitable medium drift apart over thousands
+of steps, so these gates compare statistics, as the accuracy target for long
+runs states: action-potential durations (the fused sAPD update) and the
+spiral-tip trajectory, each within 1%.
+"""
+import numpy as np
+import pytest
+
+import oracle.oracle as O
+import yolohtli_amd as Y
+from yolohtli_amd import abi_types as T
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cfg", [(T.YH_KERNEL_TILED, 8, 4, 16, 1), (T.YH_KERNEL_TILED, 6, 8, 8, 1),
+                                 (T.YH_KERNEL_DIRECT, 1, 0, 0, 1)])
+def test_fused_apd_matches_oracle(torch_cuda, cfg):
+    n, nsteps = 64, 20000
+    p = T.grid_params(n, n)
+    f = Y.initGates(n, n)
+    pl = Y.Plan(p)
+    pl.set_state(*f)
+    pl.set_kernel(*cfg)
+    pl.apd_enable(0.15)
+    for _ in range(nsteps // 1000):  # several calls: the step count carries over
+        pl.step(1000)
+    g = pl.apd_get()
+    _, st = O.run_apd(p, [x.astype(np.float64) for x in f], nsteps)
+    a_g, a_o = g["apd1"].astype(np.float64), st["apd1"]
+    both = (a_g != 0) & (a_o != 0)
+    assert both.sum() > 1000
+    assert np.mean((a_g != 0) != (a_o != 0)) < 0.01
+    rel_mean = abs(a_g[both].mean() - a_o[both].mean()) / a_o[both].mean()
+    print("cells", both.sum(), "mean APD gpu", a_g[both].mean(), "oracle", a_o[both].mean())
+    assert rel_mean < 0.01
+    assert np.quantile(np.abs(a_g[both] - a_o[both]), 0.99) < 0.01 * a_o[both].mean()
+
+
+def test_apd_with_mask_skips_holes(torch_cuda):
+    n = 64
+    p = T.grid_params(n, n, solidSwitch=1)
+    mask = Y.make_holes(n, n, 5, 3.0, 6.0, 7)
+    f = Y.initGates(n, n)
+    pl = Y.Plan(p)
+    pl.set_mask(mask)
+    pl.set_state(*f)
+    pl.apd_enable(0.15)
+    pl.step(12000)
+    g = pl.apd_get()
+    _, st = O.run_apd(p, [x.astype(np.float64) for x in f], 12000, solid=mask)
+    assert np.all(g["apd1"][mask == 0] == 0)
+    both = (g["apd1"] != 0) & (st["apd1"] != 0)
+    assert both.sum() > 100
+    assert abs(g["apd1"][both].mean() - st["apd1"][both].mean()) < 0.01 * st["apd1"][both].mean()
+
+
+def test_tip_trajectory_statistics(torch_cuda):
+    """Spiral from the reference's cross-field start, tips every 100 steps."""
+    n, warm, nsteps, every = 256, 6000, 12000, 100
+    p = T.grid_params(n, n)
+    f = Y.initGates(n, n)
+    pl = Y.Plan(p)
+    pl.set_state(*f)
+    pl.step(warm)
+    pl.clear_tips()
+    for _ in range(nsteps // every):
+        pl.step(every)
+        pl.track_tips()
+    gt = pl.read_tips()
+    X = [x.astype(np.float64) for x in f]
+    X = O.run(p, X, warm)
+    ot = []
+    c = O.derive(p)
+    for q in range(nsteps // every):
+        X = O.run(p, X, every - 1)
+        prev = X[0].copy()
+        X = O.run(p, X, 1)
+        t, _ = O.tips(p, prev, X[0], c.dt * (warm + (q + 1) * every))
+        ot.append(t)
+    ot = np.concatenate(ot)
+    print("tips gpu", len(gt), "oracle", len(ot))
+    assert len(ot) > 50
+    assert abs(len(gt) - len(ot)) <= 0.01 * len(ot) + 2
+    for k in (0, 1):
+        mg, mo = gt[:, k].mean(), ot[:, k].mean()
+        sg, so = gt[:, k].std(), ot[:, k].std()
+        print("axis", k, "mean", mg, mo, "std", sg, so)
+        assert abs(mg - mo) <= 0.01 * mo
+        assert abs(sg - so) <= 0.01 * so + 0.05
