@@ -57,30 +57,67 @@ def test_apd_with_mask_skips_holes(torch_cuda):
     assert abs(g["apd1"][both].mean() - st["apd1"][both].mean()) < 0.01 * st["apd1"][both].mean()
 
 
+def _physical(tips, past, present, n, uth=0.7):
+    """Tips whose cell has Uth between the corner extremes of both fields.
+
+    The closed-form root of tipTracker.cu:86-97 can land inside (0,1)^2 by
+    cancellation where both fields are nearly flat far below Uth; FP64 keeps
+    exponentially small tails ahead of a wave that FP32 underflows, so such
+    roots differ between the two and are no spiral tips."""
+    keep = []
+    P, Q = past.reshape(n, n), present.reshape(n, n)
+
+    for t in tips:
+        i, j = int(t[0]), int(t[1])
+        cp = P[j:j + 2, i:i + 2]
+        cq = Q[j:j + 2, i:i + 2]
+        keep.append(cp.min() < uth < cp.max() and cq.min() < uth < cq.max())
+    return tips[np.array(keep, bool)] if len(tips) else tips
+
+
+def _persistent(samples, radius=2.0):
+    """Tips that have a tip within `radius` cells in the previous or the next
+    sample.  u^n and u^{n+1} isolines are one step apart, so along a
+    translating front they nearly coincide and sporadic roots there are
+    decided by rounding; a spiral tip persists from sample to sample."""
+    out = []
+    for q, s in enumerate(samples):
+        nb = [samples[r] for r in (q - 1, q + 1) if 0 <= r < len(samples) and len(samples[r])]
+        for t in s:
+            if any(np.min(np.hypot(o[:, 0] - t[0], o[:, 1] - t[1])) <= radius for o in nb):
+                out.append(t)
+    return np.array(out).reshape(-1, 5)
+
+
 def test_tip_trajectory_statistics(torch_cuda):
-    """Spiral from the reference's cross-field start, tips every 100 steps."""
+    """Broken-wave tip from the reference's cross-field start, every 100 steps."""
     n, warm, nsteps, every = 256, 6000, 12000, 100
     p = T.grid_params(n, n)
     f = Y.initGates(n, n)
+    c = O.derive(p)
     pl = Y.Plan(p)
     pl.set_state(*f)
     pl.step(warm)
-    pl.clear_tips()
-    for _ in range(nsteps // every):
-        pl.step(every)
+    gt = []
+    for q in range(nsteps // every):
+        pl.step(every - 1)
+        prev = pl.get_state()[0]
+        pl.step(1)
+        cur = pl.get_state()[0]
+        pl.clear_tips()
         pl.track_tips()
-    gt = pl.read_tips()
+        gt.append(_physical(pl.read_tips(), prev, cur, n))
+    gt = _persistent(gt)
     X = [x.astype(np.float64) for x in f]
     X = O.run(p, X, warm)
     ot = []
-    c = O.derive(p)
     for q in range(nsteps // every):
         X = O.run(p, X, every - 1)
         prev = X[0].copy()
         X = O.run(p, X, 1)
         t, _ = O.tips(p, prev, X[0], c.dt * (warm + (q + 1) * every))
-        ot.append(t)
-    ot = np.concatenate(ot)
+        ot.append(_physical(t, prev, X[0], n))
+    ot = _persistent(ot)
     print("tips gpu", len(gt), "oracle", len(ot))
     assert len(ot) > 50
     assert abs(len(gt) - len(ot)) <= 0.01 * len(ot) + 2
