@@ -279,6 +279,44 @@ int yh_adjoint_forward(yh_adjoint* adj, const float* d_D, const double tau6[6], 
 /* d_gD: device float64 (nx*ny) receives dL/dD; gtau receives dL/dtau6. */
 int yh_adjoint_backward(yh_adjoint* adj, void* stream, double* d_gD, double gtau[6]);
 
+/* ---------------------------------------------------------- file formats */
+/* The reference's text files, written with its printf formats so its MATLAB
+ * tooling and its own loaders read them unchanged.  Host memory only. */
+
+/* dataparamcsv.csv (printParameters, printFunctions.cu:266-403): 91 lines
+ * "label,value" in the reference's positional order.  Solver fields come from
+ * p, derived terms as saveFiles.cu:140-170 and main.cu:148-155 compute them;
+ * the reference's display and recording fields carry their parameterSetup
+ * defaults (saveFiles.cu:111-225).  ic_path / results_path may be NULL ("NA");
+ * tipx, tipy: the last tip point. */
+int yh_params_write_csv(const char* path, const yh_params* p, const char* ic_path,
+                        const char* results_path, float tipx, float tipy);
+/* loadParamValues (saveFiles.cu:540-711): the second comma-separated field of
+ * each line, parsed with strtof, assigned positionally to the solver fields of
+ * p.  Fields the file does not carry (model, Fenton-Karma set, Heaviside
+ * width) keep their values.  YH_EINVAL for a file of fewer than 91 lines. */
+int yh_params_read_csv(const char* path, yh_params* p);
+/* raw_data.dat (print2D2column, printFunctions.cu:58-78): "%f %f\n" (u v) per
+ * cell, i fastest; with w non-NULL (FK) a third column.  The reader parses it
+ * back as loadData does (saveFiles.cu:512-536). */
+int yh_state_write_dat(const char* path, int nx, int ny, const float* u, const float* v,
+                       const float* w);
+int yh_state_read_dat(const char* path, int nx, int ny, float* u, float* v, float* w);
+/* dataTip.dat + dataTipSize.dat (printTip, printFunctions.cu:149-195): each
+ * tip as "%f %f %f %f %f\n" (x y vx vy t) and, when n > 0, "%d\n" with n in
+ * the size file; append != 0 appends, as every call after the first does. */
+int yh_tips_write_dat(const char* tip_path, const char* size_path, const yh_tip* tips, int n,
+                      int append);
+/* c_phi_list_sym.dat (printSym, printFunctions.cu:249-264): one line
+ * "%f %f %f %f %f %f\n" per step, c then phi: the layout yh_sr_step's
+ * c_phi_out fills. */
+int yh_sym_write_dat(const char* path, const double* c_phi, int nsteps);
+/* Obstacle mask file read at main.cu:663-681: nx*ny numbers separated by white
+ * space; cell i + nx*j is tissue when its number is > 0.5.  The writer uses
+ * MATLAB's save -ascii layout ("%16.7e" per line) of common/Hole_generator.m. */
+int yh_mask_read_dat(const char* path, int nx, int ny, uint8_t* mask);
+int yh_mask_write_dat(const char* path, int nx, int ny, const uint8_t* mask);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,15 @@ PROTOTYPES = {
     "yh_adjoint_set_targets": (I, [P, P]),
     "yh_adjoint_forward": (I, [P, P, P, P, C.POINTER(D), P]),
     "yh_adjoint_backward": (I, [P, P, P, P]),
+    "yh_params_write_csv": (I, [C.c_char_p, PARAMS_P, C.c_char_p, C.c_char_p, C.c_float,
+                                C.c_float]),
+    "yh_params_read_csv": (I, [C.c_char_p, PARAMS_P]),
+    "yh_state_write_dat": (I, [C.c_char_p, I, I, P, P, P]),
+    "yh_state_read_dat": (I, [C.c_char_p, I, I, P, P, P]),
+    "yh_tips_write_dat": (I, [C.c_char_p, C.c_char_p, P, I, I]),
+    "yh_sym_write_dat": (I, [C.c_char_p, P, I]),
+    "yh_mask_read_dat": (I, [C.c_char_p, I, I, P]),
+    "yh_mask_write_dat": (I, [C.c_char_p, I, I, P]),
 }
 
 

@@ -25,7 +25,7 @@ HIPFLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["api.cpp", "plan.cpp", "autotune.cpp", "stepper.hip", "sr.hip", "batch.hip",
-           "adjoint.hip"]
+           "adjoint.hip", "io.cpp"]
 
 
 def _compile(src):
