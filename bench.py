@@ -41,8 +41,15 @@ def parse():
     ap.add_argument("--tip-every", type=int, default=100)
     ap.add_argument("--cfg", default="auto", help="auto or K,rows,waves,cols")
     ap.add_argument("--kernel-launches", type=int, default=300)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="timed CPU-baseline seconds, split between FP32 and FP64")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sr-n", type=int, default=512)
+    ap.add_argument("--sr-steps", type=int, default=20000)
+    ap.add_argument("--no-sr", action="store_true")
+    ap.add_argument("--control-n", type=int, default=4096,
+                    help="side of the non-cache-resident control grid")
+    ap.add_argument("--no-control", action="store_true")
     ap.add_argument("--batch-tissues", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5000)
     ap.add_argument("--batch-each-steps", type=int, default=1000,
@@ -60,39 +67,48 @@ def parse():
 
 
 def cpu_baseline(p, fields, mask, seconds):
-    """The oracle's FP32 restatement of the update, OpenMP over rows."""
-    import oracle.oracle as O
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:  # pragma: no cover
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
-    n = p.nx * p.ny
-    steps = 4
-    t0 = time.perf_counter()
-    O.run(p, fields, steps, solid=mask, dtype=np.float32, nthreads=cores)
-    dt1 = time.perf_counter() - t0
-    steps = max(4, int(seconds / max(dt1 / 4, 1e-6)))
-    t0 = time.perf_counter()
-    O.run(p, fields, steps, solid=mask, dtype=np.float32, nthreads=cores)
-    el = time.perf_counter() - t0
-    return {"value": n * steps / el / 1e9, "unit": "Gcell-steps/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{p.nx}x{p.ny} same model/mask/IC, {steps} steps, FP32, "
-                      f"{cores} OpenMP threads, {el:.1f} s"}
+    """The oracle's restatement of the update as an OpenMP host loop, in a
+    child process with OMP_PROC_BIND=close and OMP_PLACES=cores: 100 warm-up
+    steps, then the median of five timed windows, FP32 and FP64."""
+    import subprocess
+    import tempfile
+    nv = len(fields)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "workload.npz")
+        np.savez(path, nx=p.nx, ny=p.ny, model=p.model,
+                 mask=mask if mask is not None else np.zeros(0, np.uint8),
+                 u=fields[0], v=fields[1], w=fields[2] if nv > 2 else np.zeros(0, np.float32))
+        env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", path,
+                            "--seconds", str(seconds)], cwd=ROOT, env=env,
+                           capture_output=True, text=True, timeout=30 * seconds + 300)
+    if r.returncode != 0:
+        raise RuntimeError("CPU baseline failed:\n" + r.stderr[-2000:])
+    c = json.loads(r.stdout.strip().splitlines()[-1])
+    f32, f64 = c["fp32"], c["fp64"]
+    return {"value": round(f32["gcell_steps_per_s"], 4), "unit": "Gcell-steps/s",
+            "cores": c["threads"], "kind": "port",
+            "sample": f"{p.nx}x{p.ny} same model/mask/IC; after 100 warm-up steps, median of "
+                      f"5 windows of {f32['steps_per_window']} steps (FP32) and "
+                      f"{f64['steps_per_window']} steps (FP64); {c['threads']} OpenMP threads, "
+                      f"OMP_PROC_BIND=close, OMP_PLACES=cores; {c['cpu']}",
+            "fp32_GBps": round(f32["GBps"], 2),
+            "fp64_gcell_steps_per_s": round(f64["gcell_steps_per_s"], 4),
+            "fp64_GBps": round(f64["GBps"], 2)}
 
 
-def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell):
-    """Average duration of the dominant kernel: `kernel_launches` back-to-back
-    launches of K fused steps each (no single-step tail, no tip kernel),
-    bracketed by two HIP events on the stream the kernel runs on."""
+def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell, launches=None):
+    """Average duration of the dominant kernel: `launches` (default
+    --kernel-launches) back-to-back launches of K fused steps each (no
+    single-step tail, no tip kernel), bracketed by two HIP events on the
+    stream the kernel runs on."""
     import torch
     from yolohtli_amd import abi_types as T
     pl.set_option(T.YH_OPT_KEEP_PREV, 0)
     K = cfg[1] if cfg[0] == T.YH_KERNEL_TILED else 1
     for _ in range(20):
         pl.step(K, stream)
-    nl = a.kernel_launches
+    nl = launches or a.kernel_launches
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -105,6 +121,89 @@ def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell):
     alg_bytes = n * bytes_per_cell  # per launch, independent of K
     achieved = alg_bytes / (t_launch_ms / 1e3) / 1e9
     return t_launch_ms, K, alg_bytes, achieved
+
+
+def symmetry_reduction(a):
+    """Symmetry-reduction configuration: 512^2 FK, cross-field start (u=1 for
+    i<32, v=w=1 for j<ny/2, 0 above), plain steps until a tip exists (the
+    count is recorded), then --sr-steps co-moving-frame steps at dt=0.01 ms,
+    each RD + tips + fused slices/inner products + 3x3 solve + 3 BFECC phases,
+    steady steps replayed from one captured hipGraph."""
+    import torch
+    import yolohtli_amd as Y
+    from yolohtli_amd import abi_types as T
+    n = a.sr_n
+    p0 = T.grid_params(n, n, model=T.YH_MODEL_3V_FK)
+    pl = Y.Plan(p0)
+    pl.set_state(*Y.initGates(n, n, u_cols=32, rest=1.0, refr=0.0, nvars=3))
+    warm, tips = 0, []
+    while warm < 20000 and not len(tips):
+        pl.step(100)
+        warm += 100
+        pl.track_tips()
+        tips = pl.read_tips()
+    st = pl.get_state()
+    pl.close()
+    if not len(tips):
+        raise RuntimeError("no spiral tip after 20000 plain steps")
+    tip0 = (float(tips[-1, 0]), float(tips[-1, 1]))
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, reduceSym=1)
+    pl = Y.Plan(p)
+    pl.set_state(*st)
+    stream = torch.cuda.current_stream()
+    pl.sr_begin(tip0, stream)
+    pl.sr_step(100, stream)  # eager first steps, graph capture, warm-up
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    pl.sr_step(a.sr_steps, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    info = pl.sr_info()
+    finite = pl.check_finite() == 0
+    pl.close()
+    rate = n * n * a.sr_steps / (ms / 1e3) / 1e9
+    log(f"SR: {a.sr_steps} steps at {n}^2 in {ms:.1f} ms = {rate:.2f} Gcell-steps/s "
+        f"({1e3 * ms / a.sr_steps:.2f} us/step); c={info['c']}")
+    if not finite:
+        raise RuntimeError("SR state is not finite")
+    # two state passes per step (RD, BFECC), 2*nvars*4 B each
+    bpc = 2 * (2 * 3 * 4)
+    return {"n": n, "model": "FK 3V", "plain_warmup_steps": warm, "steps": a.sr_steps,
+            "ms": round(ms, 3), "us_per_step": round(1e3 * ms / a.sr_steps, 3),
+            "gcell_steps_per_s": round(rate, 3), "alg_bytes_per_cell_step": bpc,
+            "GBps_alg": round(rate * bpc, 1), "graph": True,
+            "c": [float(x) for x in info["c"]], "phi": [float(x) for x in info["phi"]],
+            "singular": int(info["status"] & 1)}
+
+
+def control_grid(a, cfg):
+    """Non-cache-resident control: the headline kernel and the one-step
+    kernel on a --control-n^2 grid (FK, holes at the headline density), whose
+    state exceeds the 256 MB Infinity Cache; launch time by HIP events."""
+    import torch
+    import yolohtli_amd as Y
+    from yolohtli_amd import abi_types as T
+    n = a.control_n
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    holes = int(round(a.holes * (n / 1024.0) ** 2))
+    pl = Y.Plan(p)
+    pl.set_mask(Y.make_holes(n, n, holes, 3.0, 12.0, Y.seed(1, 1, 1)))
+    pl.set_state(*Y.init_random(n, n, Y.seed(1, 1, 2), gate=1.0, nvars=3))
+    stream = torch.cuda.current_stream()
+    out = {"n": n, "holes": holes, "state_bytes": n * n * 25}
+    for name, c in (("tiled", cfg), ("one_step", (T.YH_KERNEL_DIRECT, 1, 0, 0, 1))):
+        pl.set_kernel(*c)
+        t_ms, K, alg, ach = kernel_timing(a, pl, c, stream, n * n, 25, launches=50)
+        out[name] = {"K": K, "rows": c[2], "waves": c[3], "cols": c[4],
+                     "launch_us": round(t_ms * 1e3, 2), "achieved_GBps": round(ach, 1),
+                     "frac": round(ach / HBM_PEAK_GBPS, 4),
+                     "gcell_steps_per_s": round(n * n * K / (t_ms / 1e3) / 1e9, 2)}
+        log(f"control {n}^2 {name}: {t_ms * 1e3:.1f} us/launch K={K}, {ach:.0f} GB/s")
+    pl.close()
+    return out
 
 
 def batched(a):
@@ -302,6 +401,14 @@ def main():
         f"({n * K / (t_launch_ms / 1e3) / 1e9:.1f} Gcell-steps/s inside one launch)")
 
     pl.close()
+    control = None
+    if not a.no_control and cfg[0] == T.YH_KERNEL_TILED and a.model == "fk":
+        log("non-cache-resident control grid ...")
+        control = control_grid(a, cfg)
+    sr = None
+    if not a.no_sr:
+        log("symmetry-reduction configuration ...")
+        sr = symmetry_reduction(a)
     batch = None
     if not a.no_batch:
         log("batched configuration ...")
@@ -353,6 +460,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batch,
+        "control_grid": control,
+        "symmetry_reduction": sr,
         "adjoint": adj,
         "tips_logged": int(len(tips)),
     }

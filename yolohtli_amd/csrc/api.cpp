@@ -68,6 +68,7 @@ void make_kparams(const yh_params& p, const yh_derived& d, KParams* k) {
   k->fk_C1 = (float)(tcdt / f.tau_wp);
   k->fk_C2 = (float)(tcdt / f.tau_wm);
   k->hw = (float)p.heaviside_width;
+  k->iso = k->trx == k->try_ ? 1 : 0;
 }
 
 // splitmix64 (Steele, Lea & Flood 2014).
@@ -176,7 +177,7 @@ int yh_params_derive(const yh_params* p, yh_derived* d) {
           f.tau_v1m > 0 && f.tau_v2m > 0 && f.tau_wp > 0 && f.tau_wm > 0))
       return YH_EINVAL;
   }
-  const double pi = 3.14159265359;  // the reference's pi (typeDefinition.cuh:152)
+  const double pi = 3.14159265359;  // the reference's pi (globalVariables.cuh:27)
   d->hx = p->Lx / (p->nx - 1.0);
   d->hy = p->Ly / (p->ny - 1.0);
   const double th = p->degrad * pi / 180.0;

@@ -25,8 +25,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
 // inner (64*CW-2K) x (NW*R-2K) cells are exact; the outer ring of width K is
 // recomputed by the neighbouring workgroup.  In substep s only rows that stay
 // exact after it, s+1 <= row < NW*R-s-1, are computed (the test is uniform
-// over a wave).  Rows whose cells all have the regular stencil (c = 1,1,1,1)
-// take the unmasked formula; the others decode per-cell coefficient bytes.
+// over a wave).  With a mask every cell carries its five stencil weights in
+// registers, so masked and regular cells cost the same.
 template <int MODEL, int NW, int CW>
 struct TileLds {
   // [buffer][first/last row][wave][var][column]
@@ -48,7 +48,8 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
   auto& xch = lds.xch;
 
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  // wave-uniform: row tests below become scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wout = WR - 2 * K;
   const int hout = H - 2 * K;
   const int nx = k.nx, ny = k.ny;
@@ -64,9 +65,14 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
   }
 
   float X[NV][R][CW];
-  uint32_t cd[R][CW];   // coefficient bytes cW | cE<<8 | cN<<16 | cS<<24 (0 for holes)
+  // Masked stencil: the coefficients c in {0,1,2} of reactionDiffusion.cu:162-169
+  // per cell and direction, and their sum, as floats.  Sums of c*nb over
+  // integers c are exact for a constant field, so the update below leaves
+  // constant fields unchanged, as the reference's c*(nb-u) form does.
+  // Holes get c = 0.
+  constexpr int RM = MASKED ? R : 1;
+  float cw[RM][CW], ce[RM][CW], cn[RM][CW], cs[RM][CW], csum[RM][CW];
   uint32_t hole[CW];    // bit r: cell (r, c) is a hole
-  uint32_t irregular = 0;  // bit r: some cell of row r in this wave is not regular
 #pragma unroll
   for (int c = 0; c < CW; ++c) hole[c] = 0;
 #pragma unroll
@@ -74,7 +80,6 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
     const int gy = gy0 + r;
     const int sy = reflect_clamp(gy, ny);
     const bool flip_y = (gy < 0) || (gy >= ny);
-    bool odd = false;
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
       const size_t idx = (size_t)sy * nx + sx[c];
@@ -87,16 +92,19 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
         uint32_t w2 = q & 3u, e2 = (q >> 2) & 3u, n2 = (q >> 4) & 3u, s2 = (q >> 6) & 3u;
         if (flip_x[c]) { const uint32_t t = w2; w2 = e2; e2 = t; }
         if (flip_y) { const uint32_t t = n2; n2 = s2; s2 = t; }
-        cd[r][c] = w2 | (e2 << 8) | (n2 << 16) | (s2 << 24);
+        const int rm = MASKED ? r : 0;
+        cw[rm][c] = (float)w2;
+        ce[rm][c] = (float)e2;
+        cn[rm][c] = (float)n2;
+        cs[rm][c] = (float)s2;
+        csum[rm][c] = (float)(w2 + e2 + n2 + s2);
         hole[c] |= (h ? 1u : 0u) << r;
-        odd |= cd[r][c] != 0x01010101u;
         if (h) {
 #pragma unroll
           for (int a = 0; a < NV; ++a) X[a][r][c] = 0.0f;  // hole cells stay 0 (:521-522)
         }
       }
     }
-    if (MASKED && __any(odd)) irregular |= 1u << r;
   }
 
   // APD state of this lane's cells (only output cells are written back)
@@ -121,6 +129,8 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
 
   const int addr_w = ((lane + 63) & 63) << 2;
   const int addr_e = ((lane + 1) & 63) << 2;
+  [[maybe_unused]] const typename M::Pinned pinned = M::pin(k);
+  const float dscale = k.iso ? k.trx : 1.0f;  // the laplacian terms below carry trx otherwise
 
   for (int s = 0; s < K; ++s) {
     const int b = s & 1;
@@ -155,7 +165,8 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
 #pragma unroll
         for (int c = 0; c < CW; ++c) old[a][c] = X[a][r][c];
       if (active) {
-        float lap[ND][CW];
+        // dif[0]: u plus its diffusion term; dif[1] (2V): v's diffusion term
+        float dif[ND][CW];
 #pragma unroll
         for (int a = 0; a < ND; ++a) {
           const float wl = lane_from(old[a][CW - 1], addr_w);  // west of column 0
@@ -167,36 +178,49 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
             const float ev = c == CW - 1 ? er : old[a][c + 1];
             const float sv = prev[a][c];
             const float nv = (r < R - 1) ? X[a][r + 1][c] : north[a][c];
-            if (MASKED && ((irregular >> r) & 1u)) {
-              const uint32_t q = cd[r][c];
-              const float cW = (float)(q & 0xFFu), cE = (float)((q >> 8) & 0xFFu);
-              const float cN = (float)((q >> 16) & 0xFFu), cS = (float)(q >> 24);
-              const float lx = __builtin_fmaf(cE, ev - ctr, cW * (wv - ctr));
-              const float ly = __builtin_fmaf(cS, sv - ctr, cN * (nv - ctr));
-              lap[a][c] = __builtin_fmaf(k.try_, ly, k.trx * lx);
+            float lap_or_t;  // iso: sum c*(nb - ctr); otherwise the laplacian term
+            if (MASKED) {
+              const int rm = MASKED ? r : 0;
+              if (k.iso) {
+                float t = cs[rm][c] * sv;
+                t = __builtin_fmaf(cn[rm][c], nv, t);
+                t = __builtin_fmaf(ce[rm][c], ev, t);
+                t = __builtin_fmaf(cw[rm][c], wv, t);
+                lap_or_t = __builtin_fmaf(-csum[rm][c], ctr, t);
+              } else {
+                float tx = __builtin_fmaf(ce[rm][c], ev, cw[rm][c] * wv);
+                tx = __builtin_fmaf(-(cw[rm][c] + ce[rm][c]), ctr, tx);
+                float ty = __builtin_fmaf(cs[rm][c], sv, cn[rm][c] * nv);
+                ty = __builtin_fmaf(-(cn[rm][c] + cs[rm][c]), ctr, ty);
+                lap_or_t = __builtin_fmaf(k.try_, ty, k.trx * tx);
+              }
             } else {
-              const float lx = __builtin_fmaf(-2.0f, ctr, wv + ev);
-              const float ly = __builtin_fmaf(-2.0f, ctr, nv + sv);
-              lap[a][c] = __builtin_fmaf(k.try_, ly, k.trx * lx);
+              if (k.iso) {
+                lap_or_t = __builtin_fmaf(-4.0f, ctr, (wv + ev) + (nv + sv));
+              } else {
+                const float lx = __builtin_fmaf(-2.0f, ctr, wv + ev);
+                const float ly = __builtin_fmaf(-2.0f, ctr, nv + sv);
+                lap_or_t = __builtin_fmaf(k.try_, ly, k.trx * lx);
+              }
             }
+            dif[a][c] = a == 0 ? __builtin_fmaf(dscale, lap_or_t, ctr) : dscale * lap_or_t;
           }
         }
 #pragma unroll
         for (int c = 0; c < CW; ++c) {
-          float rr[3] = {0.f, 0.f, 0.f};
-          M::react(k, X[0][r][c], X[1][r][c], NV > 2 ? X[NV - 1][r][c] : 0.0f, rr[0], rr[1], rr[2]);
           const float u_old = X[0][r][c];
-          X[0][r][c] += lap[0][c] + rr[0];
+          if constexpr (MODEL == MFK) {
+            M::step(k, pinned, dif[0][c], X[0][r][c], X[1][r][c], X[NV - 1][r][c]);
+          } else {
+            float du, dv, dw;
+            M::react(k, X[0][r][c], X[1][r][c], 0.0f, du, dv, dw);
+            X[0][r][c] = dif[0][c] + du;
+            X[1][r][c] += __builtin_fmaf(k.rs, dif[ND - 1][c], dv);
+          }
           if (APD && !(MASKED && ((hole[c] >> r) & 1u))) {
             apd_cell(u_old, X[0][r][c], apd.count0 + s + 1, apd.dt, apd.th,
                      a_front[APD ? r : 0][c], a_back[APD ? r : 0][c], a_d1[APD ? r : 0][c],
                      a_d2[APD ? r : 0][c], a_first[APD ? r : 0][c]);
-          }
-          if (MODEL == M2V) {
-            X[1][r][c] += __builtin_fmaf(k.rs, lap[ND - 1][c], rr[1]);
-          } else {
-            X[1][r][c] += rr[1];
-            X[NV - 1][r][c] += rr[2];
           }
         }
       }

@@ -31,6 +31,7 @@ struct KParams {
   float fk_B1, fk_B2, fk_B3;         // tcdt/tau_vp, tcdt/tau_v1m, tcdt/tau_v2m
   float fk_C1, fk_C2;                // tcdt/tau_wp, tcdt/tau_wm
   float hw;                          // tanh-smoothed Heaviside width (gradient path)
+  int iso;                           // trx == try: one stencil weight for both axes
 };
 
 // Fields of the state (u, v, w); unused entries are null.
@@ -101,6 +102,8 @@ struct Model<M2V> {
     const float t3 = (u - k.gamma) * (k.beta - u);
     dv = __builtin_fmaf(-k.a2_F, v, __builtin_fmaf(k.a2_E, t3, -k.a2_G));
   }
+  struct Pinned {};
+  __device__ __forceinline__ static Pinned pin(const KParams&) { return {}; }
 };
 
 template <>
@@ -129,6 +132,48 @@ struct Model<MFK> {
     const float wo = __builtin_fmaf(-k.fk_C2, w, k.fk_C2);
     const float wc = -k.fk_C1 * w;
     dw = p ? wc : wo;
+  }
+
+  // Uniform constants held in vector registers for a whole tile: each meets
+  // another uniform operand in one instruction (a select between two
+  // constants, an fma with two constant operands), and a VALU instruction
+  // reads at most one scalar register.  The empty asm makes the values opaque
+  // so they stay in VGPRs instead of being re-materialised per cell.
+  struct Pinned {
+    float bu, B2, B3;
+  };
+  __device__ __forceinline__ static Pinned pin(const KParams& k) {
+    Pinned q{k.fk_bu, k.fk_B2, k.fk_B3};
+    asm volatile("" : "+v"(q.bu), "+v"(q.B2), "+v"(q.B3));
+    return q;
+  }
+  // One whole explicit-Euler step of a cell, the same update as react():
+  // ud = u plus its diffusion term; u <- ud - tcdt*I_ion, and v, w take their
+  // gate steps as v + b*(1-v) = fma(b, 1-v, v) (opening) and
+  // v - b*v = fma(-b, v, v) (closing), likewise w.  These keep the fixed
+  // points 1 and 0 exact and carry b at its own precision; forms with the
+  // folded constant 1-b round it to half an ulp of 1, which for b ~ 1e-3
+  // is a relative error of 1e-5 in the rate, and moves the opening fixed
+  // point of fma(v, 1-b, b) by (ulp/2)/b.
+  __device__ __forceinline__ static void step(const KParams& k, const Pinned& q, float ud,
+                                              float& u, float& v, float& w) {
+    const bool p = u >= k.u_c;
+    const bool qv = u >= k.u_v;
+    const float s1 = u - k.u_c;
+    const float s2 = __builtin_fmaf(u, -k.fk_A1, k.fk_A1);
+    const float fi = __builtin_fmaf(s1 * s2, v, -k.fk_A2);
+    const float so = -k.fk_A3 * u;
+    const float r1 = p ? fi : so;
+    const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, q.bu));
+    const float sg = __builtin_amdgcn_rcpf(ex + 1.0f);
+    u = ud + __builtin_fmaf(k.fk_A4 * w, sg, r1);
+    const float bq = qv ? q.B3 : q.B2;
+    const float vo = __builtin_fmaf(bq, 1.0f - v, v);
+    const float vc = __builtin_fmaf(-k.fk_B1, v, v);
+    v = p ? vc : vo;
+    const float wo = __builtin_fmaf(k.fk_C2, 1.0f - w, w);
+    const float wc = __builtin_fmaf(-k.fk_C1, w, w);
+    w = p ? wc : wo;
   }
 };
 
