@@ -1,19 +1,25 @@
 // sr.hip -- symmetry-reduction (co-moving frame) step on the device.
 //
-// One SR step follows display() (main.cu:894-953):
+// One SR step follows display() (main.cu:894-953) in six launches:
 //   1 RD            X -> Xo                         (reactionDiffusion_kernel)
-//   2 tips          g_past = Xo.u, g_present = X.u   (tip_wrapper, main.cu:900)
-//   3 k_sr_center   disk centre = last tip of this step, tipx0 at count 0
-//   4 k_sr_slices   slices, template and velTan in FP64, the 12 disk inner
-//                   products as per-block partial sums (slice_kernel +
-//                   trapz_kernel fused: no slice arrays, no atomics)
+//   2 k_sr_tips     g_past = Xo.u, g_present = X.u   (tip_wrapper, main.cu:900)
+//   3 k_sr_center   one workgroup: disk centre = last tip of this step, tipx0
+//                   at count 0; the step's tips move to the plan's log
+//   4 k_sr_slices   over the disk's bounding box: slices, template and velTan
+//                   in FP64, the 12 disk inner products as per-workgroup
+//                   partial sums (slice_kernel + trapz_kernel fused: no slice
+//                   arrays, no atomics)
 //   5 k_sr_solve    one workgroup: fixed-order reduction of the partials,
 //                   solve_matrix (symmetryReduction.cu:386-418), advection
-//                   parameters for Cxy_field (:45-56), phi += dt*c, tip log
-//   6 k_bf_phase1/2/3  BFECC of Xo into X (advFDBFECC.cu:18-165), one kernel
-//                   per dependent phase so no neighbour read races
-// At count 0 steps 4-5 run twice (main.cu:910-921).  Nothing leaves the
+//                   parameters for Cxy_field (:45-56), phi += dt*c
+//   6 k_bf_fused    BFECC of Xo into X (advFDBFECC.cu:18-165): the three
+//                   dependent phases through an LDS tile with a 3-cell halo,
+//                   so no neighbour read races
+// At count 0 launches 4-5 run twice (main.cu:910-921).  Nothing leaves the
 // device; a steady step is captured once into a hipGraph and replayed.
+// Results cross workgroups only at kernel boundaries: on this 8-XCD part a
+// device-scope fence writes back the whole L2 of its XCD, so a
+// last-workgroup election in every workgroup costs far more than a launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +47,7 @@ struct SrDev {
 struct SrGeom {
   int nx, ny, r2, nv;
   double hx, hy, invdx, invdy, dt, tc;
+  double dtx, dty;  // dt/hx, dt/hy (advection Courant factors)
 };
 
 constexpr int kSrTipCap = 4096;
@@ -51,23 +58,48 @@ __device__ __forceinline__ bool in_disk(const SrGeom& g, int i, int j, int cx, i
   return a * a + b * b < g.r2;
 }
 
+// ------------------------------------------------------------------- tips
+// Tips of one SR step over the cells the tip test can accept: the centred
+// disk's bounding box with a mask, the whole grid without (tip_cell).
+__global__ __launch_bounds__(256) void k_sr_tips(SrGeom g, int i0, int j0, int masked_disk,
+                                                 double Uth, const float* __restrict__ past,
+                                                 const float* __restrict__ present,
+                                                 const SrDev* st, yh_tip* tips, int* keys,
+                                                 int* count) {
+  const int i = i0 + blockIdx.x * 64 + threadIdx.x;
+  const int j = j0 + blockIdx.y * 4 + threadIdx.y;
+  if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) return;
+  // physicalTime = dt*count (main.cu:885)
+  const float t = (float)(g.dt * (double)st->count);
+  tip_cell(i, j, g.nx, g.ny, masked_disk != 0, g.r2, Uth, past, present, t, tips, keys, count,
+           kSrTipCap);
+}
+
 // ---------------------------------------------------------------- centre
-__global__ void k_sr_center(SrGeom g, SrDev* st, const yh_tip* tips, const int* keys,
-                            const int* count) {
+// One workgroup: the disk centre is the last tip of this step in (cell, root)
+// order, tipx0 at count 0 (symmetryReduction.cu:95-107), the previous centre
+// when there is no tip; then this step's tips move to the plan's log and the
+// step buffer is emptied.
+__global__ __launch_bounds__(256) void k_sr_center(SrGeom g, SrDev* st, const yh_tip* step_tips,
+                                                   const int* step_keys, int* step_count,
+                                                   yh_tip* log_tips, int* log_keys,
+                                                   int* log_count, int logcap) {
   __shared__ int best_key[256];
   __shared__ int best_slot[256];
+  __shared__ int base;
   const int t = threadIdx.x;
-  const int n = min(*count, kSrTipCap);
+  const int found = *step_count;
+  const int n = min(found, kSrTipCap);
   int bk = -1, bs = -1;
-  for (int q = t; q < n; q += blockDim.x)
-    if (keys[q] > bk) {
-      bk = keys[q];
+  for (int q = t; q < n; q += 256)
+    if (step_keys[q] > bk) {
+      bk = step_keys[q];
       bs = q;
     }
   best_key[t] = bk;
   best_slot[t] = bs;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+  for (int w = 128; w > 0; w >>= 1) {
     if (t < w && best_key[t + w] > best_key[t]) {
       best_key[t] = best_key[t + w];
       best_slot[t] = best_slot[t + w];
@@ -75,34 +107,48 @@ __global__ void k_sr_center(SrGeom g, SrDev* st, const yh_tip* tips, const int* 
     __syncthreads();
   }
   if (t == 0) {
-    st->ntips = *count;
+    st->ntips = found;
     if (st->count == 0) {
       // tipx0_d at count 0 (symmetryReduction.cu:98-100)
       st->cx = __float2int_rn(st->tip0x - (float)(g.nx / 2));
       st->cy = __float2int_rn(st->tip0y - (float)(g.ny / 2));
     } else if (best_slot[0] >= 0) {
       // last tip of this step (symmetryReduction.cu:102-103); none: keep the centre
-      const yh_tip tp = tips[best_slot[0]];
+      const yh_tip tp = step_tips[best_slot[0]];
       st->cx = __float2int_rn(tp.x - (float)(g.nx / 2));
       st->cy = __float2int_rn(tp.y - (float)(g.ny / 2));
     }
+    base = atomicAdd(log_count, found);
   }
+  __syncthreads();
+  for (int q = t; q < n; q += 256) {
+    const int slot = base + q;
+    if (slot < logcap) {
+      log_tips[slot] = step_tips[q];
+      log_keys[slot] = step_keys[q];
+    }
+  }
+  __syncthreads();
+  if (t == 0) *step_count = 0;
 }
 
 // ---------------------------------------------------------------- slices
+// Slices, template and velTan in FP64 for the cells of the disk around the
+// current centre (launched over the disk's bounding box, reach R), the 12
+// inner products as per-workgroup partial sums.
 template <int MODEL, bool MASKED>
 __global__ __launch_bounds__(kSliceBX* kSliceBY) void k_sr_slices(
-    const KParams k, const SrGeom g, CFields X, const uint8_t* __restrict__ codes,
+    const KParams k, const SrGeom g, int R, CFields X, const uint8_t* __restrict__ codes,
     const SrDev* __restrict__ st, double* __restrict__ partials) {
   using M = Model<MODEL>;
   constexpr int NV = M::NV;
-  const int i = blockIdx.x * kSliceBX + threadIdx.x;
-  const int j = blockIdx.y * kSliceBY + threadIdx.y;
+  const int cx = st->cx, cy = st->cy;
+  const int i = g.nx / 2 + cx - R + blockIdx.x * kSliceBX + threadIdx.x;
+  const int j = g.ny / 2 + cy - R + blockIdx.y * kSliceBY + threadIdx.y;
   double acc[12];
 #pragma unroll
   for (int q = 0; q < 12; ++q) acc[q] = 0.0;
-  const int cx = st->cx, cy = st->cy;
-  if (i < g.nx && j < g.ny && in_disk(g, i, j, cx, cy)) {
+  if (i >= 0 && i < g.nx && j >= 0 && j < g.ny && in_disk(g, i, j, cx, cy)) {
     const int nx = g.nx, ny = g.ny;
     const int C = i + nx * j;
     const int E = mirror(i + 1, nx) + nx * j, W = mirror(i - 1, nx) + nx * j;
@@ -180,184 +226,216 @@ __global__ __launch_bounds__(kSliceBX* kSliceBY) void k_sr_slices(
 }
 
 // ----------------------------------------------------------------- solve
-__global__ __launch_bounds__(256) void k_sr_solve(SrGeom g, SrDev* st, const double* partials,
-                                                  int nparts, int pre, int record,
-                                                  yh_tip* step_tips,
-                                                  int* step_keys, int* step_count,
-                                                  yh_tip* log_tips, int* log_keys,
-                                                  int* log_count, int logcap, double* hist,
+// One workgroup of 12 waves: wave q sums column q of the partials in a fixed
+// order (lane-strided, then a shuffle tree), so the integrals do not depend
+// on scheduling; then solve_matrix (symmetryReduction.cu:386-418), the
+// advection parameters of Cxy_field (:45-56) and phi += dt*c.
+__global__ __launch_bounds__(768) void k_sr_solve(SrGeom g, SrDev* st, const double* partials,
+                                                  int nparts, int pre, int record, double* hist,
                                                   long long histcap) {
-  __shared__ double red[256][13];
-  const int t = threadIdx.x;
-  for (int q = 0; q < 12; ++q) {
-    double s = 0.0;
-    for (int b = t; b < nparts; b += 256) s += partials[(size_t)b * 12 + q];
-    red[t][q] = s;
-  }
+  __shared__ double Is[12];
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int b = lane; b < nparts; b += 64) s += partials[(size_t)b * 12 + q];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) Is[q] = s;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w)
-      for (int q = 0; q < 12; ++q) red[t][q] += red[t + w][q];
-    __syncthreads();
+  if (threadIdx.x != 0) return;
+  double I[12];
+  for (int p = 0; p < 12; ++p) {
+    I[p] = Is[p] * (g.hx * g.hy);
+    st->I[p] = I[p];
   }
-  if (t == 0) {
-    double I[12];
-    for (int q = 0; q < 12; ++q) {
-      I[q] = red[0][q] * (g.hx * g.hy);
-      st->I[q] = I[q];
-    }
-    if (record && hist && st->count < histcap) {
-      // clist/philist push (main.cu:902-903): c and phi before this step's
-      // first solve (at count 0 that is the preliminary solve, main.cu:910-914)
-      double* h = hist + (size_t)st->count * 6;
-      for (int q = 0; q < 3; ++q) {
-        h[q] = st->c[q];
-        h[3 + q] = st->phi[q];
-      }
-    }
-    // solve_matrix, symmetryReduction.cu:386-418
-    const double cp = cos(st->phi[2]), sp = sin(st->phi[2]);
-    const double a1 = I[0] * cp + I[1] * sp, a2 = I[1] * cp - I[0] * sp, a3 = I[2];
-    const double b1 = I[3] * cp + I[4] * sp, b2 = I[4] * cp - I[3] * sp, b3 = I[5];
-    const double C1 = I[6] * cp + I[7] * sp, C2 = I[7] * cp - I[6] * sp, C3 = I[8];
-    const double d1 = I[9], d2 = I[10], d3 = I[11];
-    const double b2p = a1 / b1 * b2 - a2, b3p = a1 / b1 * b3 - a3, d2p = a1 / b1 * d2 - d1;
-    const double c2p = a1 / C1 * C2 - a2, c3p = a1 / C1 * C3 - a3, d3p = a1 / C1 * d3 - d1;
-    const double c3pp = b2p / c2p * c3p - b3p, d3pp = b2p / c2p * d3p - d2p;
-    const double x3 = d3pp / c3pp;
-    const double x2 = (d2p - b3p * x3) / b2p;
-    const double x1 = (d1 - a2 * x2 - a3 * x3) / a1;
-    if (!(isfinite(x1) && isfinite(x2) && isfinite(x3))) st->status |= 1;
-    st->c[0] = x1;
-    st->c[1] = x2;
-    st->c[2] = x3;
-    // Cxy_field_kernel (symmetryReduction.cu:45-56) with the new c, current phi
-    const double ph = st->phi[2];
-    st->aphi = ph;
-    st->ac2 = x3;
-    st->axc = -x1 * cos(ph) + x2 * sin(ph);
-    st->ayc = -x1 * sin(ph) - x2 * cos(ph);
-    if (!pre) {
-      // phi += dt*c (Euler: reduceSymStart stays set, main.cu:935-941)
-      for (int q = 0; q < 3; ++q) st->phi[q] = st->phi[q] + st->c[q] * g.dt;
+  if (record && hist && st->count < histcap) {
+    // clist/philist push (main.cu:902-903): c and phi before this step's
+    // first solve (at count 0 that is the preliminary solve, main.cu:910-914)
+    double* h = hist + (size_t)st->count * 6;
+    for (int p = 0; p < 3; ++p) {
+      h[p] = st->c[p];
+      h[3 + p] = st->phi[p];
     }
   }
+  // solve_matrix, symmetryReduction.cu:386-418
+  double sp, cp;
+  sincos(st->phi[2], &sp, &cp);
+  const double a1 = I[0] * cp + I[1] * sp, a2 = I[1] * cp - I[0] * sp, a3 = I[2];
+  const double b1 = I[3] * cp + I[4] * sp, b2 = I[4] * cp - I[3] * sp, b3 = I[5];
+  const double C1 = I[6] * cp + I[7] * sp, C2 = I[7] * cp - I[6] * sp, C3 = I[8];
+  const double d1 = I[9], d2 = I[10], d3 = I[11];
+  const double b2p = a1 / b1 * b2 - a2, b3p = a1 / b1 * b3 - a3, d2p = a1 / b1 * d2 - d1;
+  const double c2p = a1 / C1 * C2 - a2, c3p = a1 / C1 * C3 - a3, d3p = a1 / C1 * d3 - d1;
+  const double c3pp = b2p / c2p * c3p - b3p, d3pp = b2p / c2p * d3p - d2p;
+  const double x3 = d3pp / c3pp;
+  const double x2 = (d2p - b3p * x3) / b2p;
+  const double x1 = (d1 - a2 * x2 - a3 * x3) / a1;
+  if (!(isfinite(x1) && isfinite(x2) && isfinite(x3))) st->status |= 1;
+  st->c[0] = x1;
+  st->c[1] = x2;
+  st->c[2] = x3;
+  // Cxy_field_kernel (symmetryReduction.cu:45-56) with the new c and the
+  // current phi (the angle of the solve above)
+  st->aphi = st->phi[2];
+  st->ac2 = x3;
+  st->axc = -x1 * cp + x2 * sp;
+  st->ayc = -x1 * sp - x2 * cp;
   if (!pre) {
-    // move this step's tips to the plan's log, then reset the step buffer
-    __syncthreads();
-    const int n = min(*step_count, kSrTipCap);
-    __shared__ int base;
-    if (t == 0) base = atomicAdd(log_count, *step_count);
-    __syncthreads();
-    for (int q = t; q < n; q += 256) {
-      const int slot = base + q;
-      if (slot < logcap) {
-        log_tips[slot] = step_tips[q];
-        log_keys[slot] = step_keys[q];
-      }
-    }
-    __syncthreads();
-    if (t == 0) {
-      *step_count = 0;
-      st->count += 1;
-    }
+    // phi += dt*c (Euler: reduceSymStart stays set, main.cu:935-941)
+    for (int p = 0; p < 3; ++p) st->phi[p] = st->phi[p] + st->c[p] * g.dt;
+    st->count += 1;
   }
 }
 
 // ----------------------------------------------------------------- BFECC
-struct BfIdx {
-  int W, E, S, N, iW, W2, iE, E2, iS, S2, iN, N2;
-  bool sc;
-};
-
-// advFDBFECC.cu:44-66 (masked) and :112-126 (square); a neighbour outside
-// the domain counts as a hole, and each cell tests the mask of the neighbour
-// it actually uses.
+// advFDBFECC.cu:44-66 (masked) and :112-126 (square): which neighbours a cell
+// uses.  A neighbour outside the domain counts as a hole, and each cell tests
+// the mask of the neighbour it actually uses.  Bits: 0 sc (cell is tissue),
+// 1 sw, 2 se, 3 ss, 4 sn (that neighbour is usable).
 template <bool MASKED>
-__device__ __forceinline__ BfIdx bf_index(int i, int j, int nx, int ny, const uint8_t* solid) {
-  const int c = i + nx * j;
+__device__ __forceinline__ uint32_t bf_flags(int i, int j, int nx, int ny,
+                                             const uint8_t* __restrict__ solid) {
+  const int gc = i + nx * j;
   bool sc = true, sw, se, ss, sn;
   if (MASKED) {
-    sc = solid[c] != 0;
-    sw = (i > 0) && solid[c - 1];
-    se = (i < nx - 1) && solid[c + 1];
-    ss = (j > 0) && solid[c - nx];
-    sn = (j < ny - 1) && solid[c + nx];
+    sc = solid[gc] != 0;
+    sw = (i > 0) && solid[gc - 1];
+    se = (i < nx - 1) && solid[gc + 1];
+    ss = (j > 0) && solid[gc - nx];
+    sn = (j < ny - 1) && solid[gc + nx];
   } else {
     sw = i > 0;
     se = i < nx - 1;
     ss = j > 0;
     sn = j < ny - 1;
   }
+  return (sc ? 1u : 0u) | (sw ? 2u : 0u) | (se ? 4u : 0u) | (ss ? 8u : 0u) | (sn ? 16u : 0u);
+}
+
+struct BfIdx {
+  int W, E, S, N, iW, W2, iE, E2, iS, S2, iN, N2;
+};
+
+// Neighbour indices around c (row stride `stride`) for flags f.
+__device__ __forceinline__ BfIdx bf_index(uint32_t f, int c, int stride) {
+  const bool sc = f & 1u, sw = f & 2u, se = f & 4u, ss = f & 8u, sn = f & 16u;
   BfIdx r;
-  r.sc = sc;
   r.W = sc ? (sw ? c - 1 : c + 1) : c;
   r.E = sc ? (se ? c + 1 : c - 1) : c;
-  r.S = sc ? (ss ? c - nx : c + nx) : c;
-  r.N = sc ? (sn ? c + nx : c - nx) : c;
+  r.S = sc ? (ss ? c - stride : c + stride) : c;
+  r.N = sc ? (sn ? c + stride : c - stride) : c;
   r.iW = sc ? (sw ? c : c + 1) : c;
   r.W2 = sc ? (sw ? c - 1 : c) : c;
   r.iE = sc ? (se ? c : c - 1) : c;
   r.E2 = sc ? (se ? c + 1 : c) : c;
-  r.iS = sc ? (ss ? c : c + nx) : c;
-  r.S2 = sc ? (ss ? c - nx : c) : c;
-  r.iN = sc ? (sn ? c : c - nx) : c;
-  r.N2 = sc ? (sn ? c + nx : c) : c;
+  r.iS = sc ? (ss ? c : c + stride) : c;
+  r.S2 = sc ? (ss ? c - stride : c) : c;
+  r.iN = sc ? (sn ? c : c - stride) : c;
+  r.N2 = sc ? (sn ? c + stride : c) : c;
   return r;
 }
 
-struct BfCoef {
-  float Rx, Ry;
-  bool px, py;
-};
+// The three dependent BFECC phases (advFDBFECC.cu:128-144) in one launch:
+// each phase reads only the previous phase at distance 1, so a workgroup
+// loads its kBfTX x kBfTY output tile with a halo of 3 into LDS and computes
+// phase p on the tile grown by 3 - p cells.  Each cell's advection
+// coefficients (Cxy_field_kernel, symmetryReduction.cu:45-56, negated as at
+// advFDBFECC.cu:30-31) and neighbour flags are computed once.  Cells outside
+// the domain are never read: the boundary rules substitute an inside
+// neighbour.
+constexpr int kBfTX = 64, kBfTY = 8, kBfHalo = 3;
+constexpr int kBfLW = kBfTX + 2 * kBfHalo, kBfLH = kBfTY + 2 * kBfHalo;
+constexpr int kBfLN = kBfLW * kBfLH;
 
-template <bool MASKED>
-__device__ __forceinline__ BfCoef bf_coef(const SrGeom& g, const SrDev* st, int i, int j,
-                                          const uint8_t* solid) {
-  double ax = g.hy * (double)j * st->ac2 + st->axc;
-  double ay = -g.hx * (double)i * st->ac2 + st->ayc;
-  if (MASKED && !solid[i + g.nx * j]) ax = ay = 0.0;  // Cxy_field_kernel :50-51
-  const double cx = -ax, cy = -ay;                      // advFDBFECC.cu:30-31
-  BfCoef r;
-  r.Rx = (float)(fabs(cx) * g.dt / g.hx);
-  r.Ry = (float)(fabs(cy) * g.dt / g.hy);
-  r.px = cx > 0.0;
-  r.py = cy > 0.0;
-  return r;
-}
-
-template <bool MASKED>
-__global__ __launch_bounds__(256) void k_bf_phase(int phase, SrGeom g, const SrDev* st,
-                                                  CFields in, Fields uf, Fields ue, Fields out,
-                                                  const uint8_t* __restrict__ solid) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= g.nx || j >= g.ny) return;
-  const int k = i + g.nx * j;
-  const BfIdx x = bf_index<MASKED>(i, j, g.nx, g.ny, solid);
-  const BfCoef cf = bf_coef<MASKED>(g, st, i, j, solid);
-  const float tc = (float)g.tc;
-  for (int a = 0; a < g.nv; ++a) {
-    if (phase == 1) {  // forward step (advFDBFECC.cu:128-131)
-      const float* q = in.f[a];
-      const float FDx = cf.px ? q[k] - q[x.W] : q[k] - q[x.E];
-      const float FDy = cf.py ? q[k] - q[x.S] : q[k] - q[x.N];
-      uf.f[a][k] = q[k] - tc * (cf.Rx * FDx + cf.Ry * FDy);
-    } else if (phase == 2) {  // backward step and error correction (:134-139)
-      const float* f = uf.f[a];
-      const float FDx = cf.px ? f[x.iE] - f[x.E2] : f[x.iW] - f[x.W2];
-      const float FDy = cf.py ? f[x.iN] - f[x.N2] : f[x.iS] - f[x.S2];
-      const float ub = f[k] - tc * (cf.Rx * FDx + cf.Ry * FDy);
-      const float g0 = in.f[a][k];
-      ue.f[a][k] = g0 - 0.5f * (ub - g0);
-    } else {  // forward step from the corrected field (:141-144)
-      const float* e = ue.f[a];
-      const float FDx = cf.px ? e[k] - e[x.W] : e[k] - e[x.E];
-      const float FDy = cf.py ? e[k] - e[x.S] : e[k] - e[x.N];
-      const float o = e[k] - tc * (cf.Rx * FDx + cf.Ry * FDy);
-      out.f[a][k] = x.sc ? o : 0.0f;
+// One phase over its region (the output tile grown by 3 - PH cells), with a
+// fixed trip count so the compiler keeps several cells' LDS reads in flight.
+template <int PH, int NV>
+__device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[kBfLN],
+                                         float (*s_f)[kBfLN],
+                                         const float* s_rx, const float* s_ry,
+                                         const uint32_t* s_fl, Fields out, int i0, int j0,
+                                         int nx) {
+  constexpr int W = kBfLW - 2 * PH, H = kBfLH - 2 * PH, N = W * H, IT = (N + 255) / 256;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int l = tid + it * 256;
+    if (l < N) {
+      const int L = (PH + l % W) + kBfLW * (PH + l / W);
+      const uint32_t fl = s_fl[L];
+      if (fl & 128u) {  // inside the domain
+        const BfIdx x = bf_index(fl, L, kBfLW);
+        const float Rx = s_rx[L], Ry = s_ry[L];
+        const bool px = fl & 32u, py = fl & 64u;
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+          if (PH == 1) {  // forward step (advFDBFECC.cu:128-131)
+            const float* q = s_q[a];
+            const float FDx = px ? q[L] - q[x.W] : q[L] - q[x.E];
+            const float FDy = py ? q[L] - q[x.S] : q[L] - q[x.N];
+            s_f[a][L] = q[L] - tc * (Rx * FDx + Ry * FDy);
+          } else if (PH == 2) {  // backward step and error correction (:134-139)
+            const float* f = s_f[a];
+            const float FDx = px ? f[x.iE] - f[x.E2] : f[x.iW] - f[x.W2];
+            const float FDy = py ? f[x.iN] - f[x.N2] : f[x.iS] - f[x.S2];
+            const float ub = f[L] - tc * (Rx * FDx + Ry * FDy);
+            const float g0 = s_q[a][L];
+            s_q[a][L] = g0 - 0.5f * (ub - g0);  // ue over the input: only cell L reads it
+          } else {  // forward step from the corrected field (:141-144)
+            const float* e = s_q[a];
+            const float FDx = px ? e[L] - e[x.W] : e[L] - e[x.E];
+            const float FDy = py ? e[L] - e[x.S] : e[L] - e[x.N];
+            const float o = e[L] - tc * (Rx * FDx + Ry * FDy);
+            const int gi = i0 + L % kBfLW, gj = j0 + L / kBfLW;
+            out.f[a][gi + (size_t)nx * gj] = (fl & 1u) ? o : 0.0f;
+          }
+        }
+      }
     }
   }
+}
+
+template <bool MASKED, int NV>
+__global__ __launch_bounds__(256) void k_bf_fused(SrGeom g, const SrDev* st, CFields in,
+                                                  Fields out, const uint8_t* __restrict__ solid) {
+  __shared__ float s_q[NV][kBfLN];  // the state advected, then the corrected field ue
+  __shared__ float s_f[NV][kBfLN];  // forward step uf
+  __shared__ float s_rx[kBfLN], s_ry[kBfLN];
+  __shared__ uint32_t s_fl[kBfLN];  // bf_flags | px << 5 | py << 6 | inside << 7
+  const int tid = threadIdx.x;
+  const int nx = g.nx, ny = g.ny;
+  const int i0 = blockIdx.x * kBfTX - kBfHalo, j0 = blockIdx.y * kBfTY - kBfHalo;
+  const double ac2 = st->ac2, axc = st->axc, ayc = st->ayc;
+  constexpr int IT = (kBfLN + 255) / 256;
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int l = tid + it * 256;
+    if (l < kBfLN) {
+      const int gi = i0 + l % kBfLW, gj = j0 + l / kBfLW;
+      const bool ok = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+#pragma unroll
+      for (int a = 0; a < NV; ++a) s_q[a][l] = ok ? in.f[a][gi + (size_t)nx * gj] : 0.0f;
+      uint32_t fl = 0;
+      float rx = 0.0f, ry = 0.0f;
+      if (ok) {
+        fl = bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u;
+        double ax = g.hy * (double)gj * ac2 + axc;
+        double ay = -g.hx * (double)gi * ac2 + ayc;
+        if (MASKED && !(fl & 1u)) ax = ay = 0.0;  // Cxy_field_kernel :50-51
+        const double cx = -ax, cy = -ay;          // advFDBFECC.cu:30-31
+        rx = (float)(fabs(cx) * g.dtx);
+        ry = (float)(fabs(cy) * g.dty);
+        fl |= (cx > 0.0 ? 32u : 0u) | (cy > 0.0 ? 64u : 0u);
+      }
+      s_rx[l] = rx;
+      s_ry[l] = ry;
+      s_fl[l] = fl;
+    }
+  }
+  __syncthreads();
+  const float tc = (float)g.tc;
+  bf_phase<1, NV>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  __syncthreads();
+  bf_phase<2, NV>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  __syncthreads();
+  bf_phase<3, NV>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
 }
 
 // ----------------------------------------------------------------- host
@@ -370,8 +448,7 @@ struct yh_sr {
   yh_tip* step_tips = nullptr;
   int* step_keys = nullptr;
   int* step_count = nullptr;
-  float* uf[3] = {nullptr, nullptr, nullptr};
-  float* ue[3] = {nullptr, nullptr, nullptr};
+  int reach = 0;        // largest R with R^2 < r2: the disks span (2R+1)^2 cells
   double* hist = nullptr;
   long long histcap = 0;
   bool begun = false;
@@ -380,7 +457,6 @@ struct yh_sr {
   hipGraphExec_t exec = nullptr;
   hipStream_t cap_stream = nullptr;
   int graph_cur = -1;
-  double* hist_graph = nullptr;
 };
 
 namespace yh {
@@ -395,10 +471,6 @@ void sr_free(yh_sr* sr) {
   if (sr->step_tips) (void)hipFree(sr->step_tips);
   if (sr->step_keys) (void)hipFree(sr->step_keys);
   if (sr->step_count) (void)hipFree(sr->step_count);
-  for (int a = 0; a < 3; ++a) {
-    if (sr->uf[a]) (void)hipFree(sr->uf[a]);
-    if (sr->ue[a]) (void)hipFree(sr->ue[a]);
-  }
   if (sr->hist) (void)hipFree(sr->hist);
   delete sr;
 }
@@ -415,15 +487,24 @@ static SrGeom sr_geom(const yh_plan* pl) {
   g.invdy = pl->d.invdy;
   g.dt = pl->d.dt;
   g.tc = pl->p.tc;
+  g.dtx = g.dt / g.hx;
+  g.dty = g.dt / g.hy;
   return g;
 }
 
-hipError_t launch_tips_dev(const KParams& k, int masked_disk, int r2, double Uth,
-                           const float* past, const float* present, double dt,
-                           const long long* d_count, yh_tip* tips, int* keys, int* count, int cap,
-                           hipStream_t s);
+// Largest R with R^2 < r2 (0 when r2 <= 1): a disk a^2 + b^2 < r2 lies in
+// the (2R+1)^2 box around its centre.
+static int disk_reach(int r2) {
+  if (r2 <= 1) return 0;
+  int R = (int)std::sqrt((double)r2);
+  while (R > 0 && (long long)R * R >= r2) --R;
+  while ((long long)(R + 1) * (R + 1) < r2) ++R;
+  return R;
+}
 
-// Enqueue one SR step (pre: the count-0 double solve) on stream s.
+// Enqueue one SR step (first: the count-0 double solve) on stream s:
+//   RD X -> Xo; tips; centre; (slices, solve) once, twice at count 0;
+//   BFECC of Xo back into X.  Six launches in a steady step.
 static int sr_enqueue(yh_plan* pl, bool first, hipStream_t s) {
   yh_sr* sr = pl->sr;
   const SrGeom g = sr_geom(pl);
@@ -433,44 +514,51 @@ static int sr_enqueue(yh_plan* pl, bool first, hipStream_t s) {
   Fields Xo{{pl->buf[b ^ 1][0], pl->buf[b ^ 1][1], pl->buf[b ^ 1][2]}};
   CFields Xoc{{Xo.f[0], Xo.f[1], Xo.f[2]}};
   Fields Xw{{pl->buf[b][0], pl->buf[b][1], pl->buf[b][2]}};
+  const int R = sr->reach, box = 2 * R + 1;
   // 1 RD
   YH_HIP(launch_direct_codes(pl->p.model, pl->k, X, Xo, masked ? pl->codes_d : nullptr, s));
-  // 2 tips: g_past = RD output, g_present = old state (main.cu:900, tipTracker.cu:586)
-  YH_HIP(launch_tips_dev(pl->k, pl->p.solidSwitch, g.r2, pl->p.Uth, Xo.f[0], X.f[0], pl->d.dt,
-                         &sr->st->count, sr->step_tips, sr->step_keys, sr->step_count, kSrTipCap,
-                         s));
-  // 3 centre
+  // 2 tips (g_past = RD output, g_present = old state: main.cu:900, tipTracker.cu:586);
+  //   with a mask only the centred disk's box can hold tips
+  {
+    const int i0 = masked ? g.nx / 2 - R : 0, j0 = masked ? g.ny / 2 - R : 0;
+    const int w = masked ? box : g.nx, h = masked ? box : g.ny;
+    hipLaunchKernelGGL(k_sr_tips, dim3((w + 63) / 64, (h + 3) / 4), dim3(64, 4), 0, s, g, i0, j0,
+                       (int)masked, pl->p.Uth, (const float*)Xo.f[0], X.f[0], sr->st,
+                       sr->step_tips, sr->step_keys, sr->step_count);
+    YH_HIP(hipGetLastError());
+  }
+  // 3 centre, tips to the log
   hipLaunchKernelGGL(k_sr_center, dim3(1), dim3(256), 0, s, g, sr->st, sr->step_tips,
-                     sr->step_keys, sr->step_count);
+                     sr->step_keys, sr->step_count, pl->tips_d, pl->keys_d, pl->tipcount_d,
+                     pl->tipcap);
   YH_HIP(hipGetLastError());
-  dim3 sb(kSliceBX, kSliceBY), sg((g.nx + kSliceBX - 1) / kSliceBX, (g.ny + kSliceBY - 1) / kSliceBY);
+  // 4 slices + inner products over the moving disk's box, 5 solve
+  dim3 sb(kSliceBX, kSliceBY), sg((box + kSliceBX - 1) / kSliceBX, (box + kSliceBY - 1) / kSliceBY);
   for (int pass = first ? 0 : 1; pass < 2; ++pass) {
     const int pre = pass == 0;
     const int record = pass == (first ? 0 : 1);
-    // 4 slices + inner products
     if (pl->p.model == M2V) {
-      if (masked) hipLaunchKernelGGL((k_sr_slices<M2V, true>), sg, sb, 0, s, pl->k, g, X, pl->codes_d, sr->st, sr->partials);
-      else hipLaunchKernelGGL((k_sr_slices<M2V, false>), sg, sb, 0, s, pl->k, g, X, pl->codes_d, sr->st, sr->partials);
+      if (masked) hipLaunchKernelGGL((k_sr_slices<M2V, true>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
+      else hipLaunchKernelGGL((k_sr_slices<M2V, false>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
     } else {
-      if (masked) hipLaunchKernelGGL((k_sr_slices<MFK, true>), sg, sb, 0, s, pl->k, g, X, pl->codes_d, sr->st, sr->partials);
-      else hipLaunchKernelGGL((k_sr_slices<MFK, false>), sg, sb, 0, s, pl->k, g, X, pl->codes_d, sr->st, sr->partials);
+      if (masked) hipLaunchKernelGGL((k_sr_slices<MFK, true>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
+      else hipLaunchKernelGGL((k_sr_slices<MFK, false>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
     }
     YH_HIP(hipGetLastError());
-    // 5 solve
-    hipLaunchKernelGGL(k_sr_solve, dim3(1), dim3(256), 0, s, g, sr->st, sr->partials, sr->nparts,
-                       pre, record, sr->step_tips, sr->step_keys, sr->step_count, pl->tips_d,
-                       pl->keys_d,
-                       pl->tipcount_d, pl->tipcap, sr->hist, sr->histcap);
+    hipLaunchKernelGGL(k_sr_solve, dim3(1), dim3(768), 0, s, g, sr->st, sr->partials,
+                       sr->nparts, pre, record, sr->hist, sr->histcap);
     YH_HIP(hipGetLastError());
   }
-  // 6 BFECC of Xo back into X
-  dim3 bb(64, 4), bg((g.nx + 63) / 64, (g.ny + 3) / 4);
-  Fields uf{{sr->uf[0], sr->uf[1], sr->uf[2]}}, ue{{sr->ue[0], sr->ue[1], sr->ue[2]}};
-  for (int ph = 1; ph <= 3; ++ph) {
-    if (masked) hipLaunchKernelGGL((k_bf_phase<true>), bg, bb, 0, s, ph, g, sr->st, Xoc, uf, ue, Xw, pl->solid_d);
-    else hipLaunchKernelGGL((k_bf_phase<false>), bg, bb, 0, s, ph, g, sr->st, Xoc, uf, ue, Xw, pl->solid_d);
-    YH_HIP(hipGetLastError());
+  // 6 BFECC of Xo back into X, the three phases in one launch
+  dim3 bg((g.nx + kBfTX - 1) / kBfTX, (g.ny + kBfTY - 1) / kBfTY);
+  if (g.nv == 3) {
+    if (masked) hipLaunchKernelGGL((k_bf_fused<true, 3>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
+    else hipLaunchKernelGGL((k_bf_fused<false, 3>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
+  } else {
+    if (masked) hipLaunchKernelGGL((k_bf_fused<true, 2>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
+    else hipLaunchKernelGGL((k_bf_fused<false, 2>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
   }
+  YH_HIP(hipGetLastError());
   return YH_OK;
 }
 
@@ -489,17 +577,15 @@ int yh_sr_begin(yh_plan* pl, float tip0x, float tip0y, void* stream) {
     yh_sr* sr = new (std::nothrow) yh_sr();
     if (!sr) return YH_ENOMEM;
     pl->sr = sr;
-    const int gx = (pl->p.nx + kSliceBX - 1) / kSliceBX, gy = (pl->p.ny + kSliceBY - 1) / kSliceBY;
+    sr->reach = disk_reach(pl->p.tipOffsetX * pl->p.tipOffsetY);
+    const int box = 2 * sr->reach + 1;
+    const int gx = (box + kSliceBX - 1) / kSliceBX, gy = (box + kSliceBY - 1) / kSliceBY;
     sr->nparts = gx * gy;
     YH_HIP(hipMalloc(&sr->st, sizeof(SrDev)));
     YH_HIP(hipMalloc(&sr->partials, sizeof(double) * 12 * sr->nparts));
     YH_HIP(hipMalloc(&sr->step_tips, sizeof(yh_tip) * kSrTipCap));
     YH_HIP(hipMalloc(&sr->step_keys, sizeof(int) * kSrTipCap));
     YH_HIP(hipMalloc(&sr->step_count, sizeof(int)));
-    for (int a = 0; a < pl->nv; ++a) {
-      YH_HIP(hipMalloc(&sr->uf[a], sizeof(float) * pl->n));
-      YH_HIP(hipMalloc(&sr->ue[a], sizeof(float) * pl->n));
-    }
   }
   yh_sr* sr = pl->sr;
   SrDev h;

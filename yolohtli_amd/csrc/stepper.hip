@@ -265,66 +265,16 @@ hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, i
 }
 
 // -------------------------------------------------------------------- tips
-// spiralTip_kernel + tipRecordPlane (tipTracker.cu:20-241) in FP64 from the
-// FP32 fields, appending {x, y, 0, 0, t} and the key 2*cell+root.
-__device__ __forceinline__ void tip_record(int i, int j, float tx, float ty, float t, int key,
-                                           yh_tip* tips, int* keys, int* count, int cap) {
-  if ((tx > 0.0f) && (tx < 1.0f) && (ty > 0.0f) && (ty < 1.0f)) {
-    const int slot = atomicAdd(count, 1);
-    if (slot < cap) {
-      yh_tip d;
-      d.x = (float)i + tx;
-      d.y = (float)j + ty;
-      d.vx = 0.0f;
-      d.vy = 0.0f;
-      d.t = t;
-      tips[slot] = d;
-      keys[slot] = key;
-    }
-  }
-}
-
 __global__ void k_tips(int nx, int ny, int masked_disk, int r2, double Uth,
                        const float* __restrict__ past, const float* __restrict__ present,
                        float t_host, const long long* d_count, double dt, yh_tip* tips,
                        int* keys, int* count, int cap) {
-#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= nx || j >= ny) return;
   // physicalTime = dt*count (main.cu:885), from the device step counter when given
   const float t = d_count ? (float)(dt * (double)(*d_count)) : t_host;
-  bool inside;
-  if (masked_disk) {
-    const int ic = i - nx / 2, jc = j - ny / 2;
-    inside = (ic * ic + jc * jc) < r2;
-  } else {
-    inside = (i >= 1) && (i < nx - 2) && (j >= 1) && (j < ny - 2);
-  }
-  if (!inside) return;
-  const int s0 = i + nx * j;
-  const int sx = (i < nx - 1) ? s0 + 1 : s0;
-  const int sy = (j < ny - 1) ? s0 + nx : s0;
-  const int sxy = ((j < ny - 1) && (i < nx - 1)) ? s0 + nx + 1 : s0;
-  const double x1 = present[s0], x2 = present[sx], x4 = present[sy], x3 = present[sxy];
-  const double y1 = past[s0], y2 = past[sx], y4 = past[sy], y3 = past[sxy];
-  const double x3y1 = x3 * y1, x4y1 = x4 * y1, x3y2 = x3 * y2, x4y2 = x4 * y2;
-  const double x1y3 = x1 * y3, x2y3 = x2 * y3, x1y4 = x1 * y4, x2y4 = x2 * y4;
-  const double x2y1 = x2 * y1, x1y2 = x1 * y2, x4y3 = x4 * y3, x3y4 = x3 * y4;
-  const double den1 = 2.0 * (x3y1 - x4y1 - x3y2 + x4y2 - x1y3 + x2y3 + x1y4 - x2y4);
-  const double den2 = 2.0 * (x2y1 - x3y1 - x1y2 + x4y2 + x1y3 - x4y3 - x2y4 + x3y4);
-  const double ctn1 = x1 - x2 + x3 - x4 - y1 + y2 - y3 + y4;
-  const double ctn2 = x3y1 - 2.0 * x4y1 + x4y2 - x1y3 + 2.0 * x1y4 - x2y4;
-  const double disc = sqrt(4.0 * (x3y1 - x3y2 - x4y1 + x4y2 - x1y3 + x1y4 + x2y3 - x2y4)
-                           * (x4y1 - x1y4 + Uth * (x1 - x4 - y1 + y4))
-                           + (-ctn2 + Uth * ctn1) * (-ctn2 + Uth * ctn1));
-  const double px = ctn2 - Uth * ctn1;
-  const double py = Uth * ctn1 - x3y1 + x4y2 + x1y3 - x2y4 + 2.0 * (x2y1 - x1y2);
-  if (!masked_disk && !(disc >= 0.0)) return;
-  tip_record(i, j, (float)((px + disc) / den1), (float)((py + disc) / den2), t, 2 * s0, tips,
-             keys, count, cap);
-  tip_record(i, j, (float)((px - disc) / den1), (float)((py - disc) / den2), t, 2 * s0 + 1, tips,
-             keys, count, cap);
+  tip_cell(i, j, nx, ny, masked_disk != 0, r2, Uth, past, present, t, tips, keys, count, cap);
 }
 
 hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int tipOffsetY,
@@ -334,16 +284,6 @@ hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int ti
   hipLaunchKernelGGL(k_tips, g, b, 0, s, k.nx, k.ny, masked_disk, tipOffsetX * tipOffsetY, Uth,
                      past, present, (float)time, (const long long*)nullptr, 0.0, tips, keys,
                      count, cap);
-  return hipGetLastError();
-}
-
-hipError_t launch_tips_dev(const KParams& k, int masked_disk, int r2, double Uth,
-                           const float* past, const float* present, double dt,
-                           const long long* d_count, yh_tip* tips, int* keys, int* count, int cap,
-                           hipStream_t s) {
-  dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
-  hipLaunchKernelGGL(k_tips, g, b, 0, s, k.nx, k.ny, masked_disk, r2, Uth, past, present, 0.0f,
-                     d_count, dt, tips, keys, count, cap);
   return hipGetLastError();
 }
 

@@ -189,6 +189,69 @@ __device__ __forceinline__ int reflect_clamp(int k, int n) {
   return m < 0 ? 0 : (m >= n ? n - 1 : m);
 }
 
+// ------------------------------------------------------------------- tips
+// spiralTip_kernel + tipRecordPlane (tipTracker.cu:20-241) in FP64 from the
+// FP32 fields, appending {x, y, 0, 0, t} and the key 2*cell+root.
+__device__ __forceinline__ void tip_record(int i, int j, float tx, float ty, float t, int key,
+                                           yh_tip* tips, int* keys, int* count, int cap) {
+  if ((tx > 0.0f) && (tx < 1.0f) && (ty > 0.0f) && (ty < 1.0f)) {
+    const int slot = atomicAdd(count, 1);
+    if (slot < cap) {
+      yh_tip d;
+      d.x = (float)i + tx;
+      d.y = (float)j + ty;
+      d.vx = 0.0f;
+      d.vy = 0.0f;
+      d.t = t;
+      tips[slot] = d;
+      keys[slot] = key;
+    }
+  }
+}
+
+// Tips of cell (i, j) between the isolines u = Uth of past and present.
+// masked_disk (the solidSwitch branch, tipTracker.cu:47-51): only cells of
+// the disk (i-nx/2)^2 + (j-ny/2)^2 < r2, both roots recorded without a
+// discriminant test (:92-117); otherwise the interior 1 <= i < nx-2,
+// 1 <= j < ny-2, roots recorded when disc >= 0 (:126-197).
+__device__ __forceinline__ void tip_cell(int i, int j, int nx, int ny, bool masked_disk, int r2,
+                                         double Uth, const float* __restrict__ past,
+                                         const float* __restrict__ present, float t,
+                                         yh_tip* tips, int* keys, int* count, int cap) {
+#pragma clang fp contract(off)
+  bool inside;
+  if (masked_disk) {
+    const int ic = i - nx / 2, jc = j - ny / 2;
+    inside = (ic * ic + jc * jc) < r2;
+  } else {
+    inside = (i >= 1) && (i < nx - 2) && (j >= 1) && (j < ny - 2);
+  }
+  if (!inside) return;
+  const int s0 = i + nx * j;
+  const int sx = (i < nx - 1) ? s0 + 1 : s0;
+  const int sy = (j < ny - 1) ? s0 + nx : s0;
+  const int sxy = ((j < ny - 1) && (i < nx - 1)) ? s0 + nx + 1 : s0;
+  const double x1 = present[s0], x2 = present[sx], x4 = present[sy], x3 = present[sxy];
+  const double y1 = past[s0], y2 = past[sx], y4 = past[sy], y3 = past[sxy];
+  const double x3y1 = x3 * y1, x4y1 = x4 * y1, x3y2 = x3 * y2, x4y2 = x4 * y2;
+  const double x1y3 = x1 * y3, x2y3 = x2 * y3, x1y4 = x1 * y4, x2y4 = x2 * y4;
+  const double x2y1 = x2 * y1, x1y2 = x1 * y2, x4y3 = x4 * y3, x3y4 = x3 * y4;
+  const double den1 = 2.0 * (x3y1 - x4y1 - x3y2 + x4y2 - x1y3 + x2y3 + x1y4 - x2y4);
+  const double den2 = 2.0 * (x2y1 - x3y1 - x1y2 + x4y2 + x1y3 - x4y3 - x2y4 + x3y4);
+  const double ctn1 = x1 - x2 + x3 - x4 - y1 + y2 - y3 + y4;
+  const double ctn2 = x3y1 - 2.0 * x4y1 + x4y2 - x1y3 + 2.0 * x1y4 - x2y4;
+  const double disc = sqrt(4.0 * (x3y1 - x3y2 - x4y1 + x4y2 - x1y3 + x1y4 + x2y3 - x2y4)
+                           * (x4y1 - x1y4 + Uth * (x1 - x4 - y1 + y4))
+                           + (-ctn2 + Uth * ctn1) * (-ctn2 + Uth * ctn1));
+  const double px = ctn2 - Uth * ctn1;
+  const double py = Uth * ctn1 - x3y1 + x4y2 + x1y3 - x2y4 + 2.0 * (x2y1 - x1y2);
+  if (!masked_disk && !(disc >= 0.0)) return;
+  tip_record(i, j, (float)((px + disc) / den1), (float)((py + disc) / den2), t, 2 * s0, tips,
+             keys, count, cap);
+  tip_record(i, j, (float)((px - disc) / den1), (float)((py - disc) / den2), t, 2 * s0 + 1, tips,
+             keys, count, cap);
+}
+
 // Stencil code of one cell, built once per mask: 2-bit coefficients
 // cW | cE<<2 | cN<<4 | cS<<6 of reactionDiffusion.cu:162-169 written as
 // sum_dir c_dir*(u_dir - u); 0xFF marks a hole.
