@@ -28,8 +28,8 @@ extern "C" {
 enum {
   YH_OK = 0,
   YH_EINVAL = 1,       /* bad argument (NULL pointer, size, range) */
-  YH_EUNSUPPORTED = 2, /* branch outside the built surface: Dirichlet, RK2/RK4,
-                          lap4, anisotropic cross term */
+  YH_EUNSUPPORTED = 2, /* the entry point runs only the standard branch (Neumann,
+                          Euler, 5-point) and the parameters select another */
   YH_EHIP = 3,         /* the HIP runtime reported an error */
   YH_ENOMEM = 4,       /* device or host allocation failed */
   YH_EUNSTABLE = 5,    /* explicit Euler diffusion number rx+ry > 0.5 */
@@ -57,7 +57,14 @@ typedef struct yh_fk_params {
 
 /* Solver subset of the reference paramVar (typeDefinition.cuh:35-125), same
  * field names and meaning.  Derived quantities follow saveFiles.cu:144-170 and
- * main.cu:148-155: hx = Lx/(nx-1); dt is halved when reduceSym is set. */
+ * main.cu:148-155: hx = Lx/(nx-1); dt is halved when reduceSym is set.
+ * Branches (reactionDiffusion.cu:36-561): neumannBC 1 (mirror) or 0
+ * (Dirichlet, boundaryVal); timeIntOrder 1 (Euler), 2 (RK2) or 4 (RK4); lap4
+ * (4th-order compact Laplacian) and anisotropy (cross term from degrad,
+ * diff_par != diff_per) act on the square domain, and with Dirichlet the
+ * anisotropic term also with a mask; with a mask and Neumann both have no
+ * effect, as in the reference.  The standard branch (Neumann, Euler, 5-point)
+ * runs on the temporally blocked kernels; the others on a per-stage path. */
 typedef struct yh_params {
   int nx, ny;
   double Lx, Ly;                 /* cm */
@@ -68,7 +75,7 @@ typedef struct yh_params {
   double tc, alpha, beta, gamma, delta, eps, mu, theta; /* 2V reaction */
   int model;                     /* YH_MODEL_2V_REF or YH_MODEL_3V_FK */
   int neumannBC, solidSwitch, gateDiff, reduceSym;
-  int timeIntOrder, lap4, anisotropy; /* only 1, 0, 0 are built */
+  int timeIntOrder, lap4, anisotropy; /* see above */
   double boundaryVal, Uth;
   int tipOffsetX, tipOffsetY;
   yh_fk_params fk;
@@ -81,11 +88,11 @@ typedef struct yh_derived {
 } yh_derived;
 
 /* Defaults of the reference parameterSetup (saveFiles.cu:105-231), with the
- * built branches selected: Euler (timeIntOrder=1), 5-point (lap4=0). */
+ * standard branch selected: Euler (timeIntOrder=1), 5-point (lap4=0). */
 void yh_params_default(yh_params* p);
 /* Fenton-Karma 1998 Beeler-Reuter parameter set. */
 void yh_fk_params_default(yh_fk_params* fk);
-/* Validate and derive; returns YH_EUNSUPPORTED for unbuilt branches. */
+/* Validate and derive (YH_EINVAL for a timeIntOrder other than 1, 2, 4). */
 int yh_params_derive(const yh_params* p, yh_derived* d);
 
 /* ------------------------------------------------------------- workloads */
@@ -105,13 +112,21 @@ int yh_make_holes(int nx, int ny, int nholes, double rmin, double rmax,
                   uint64_t seed, uint8_t* mask);
 
 /* ---------------------------------------------- single-step device wrapper */
-/* One explicit-Euler step on caller-owned device arrays.  Replaces
+/* One time step on caller-owned device arrays.  Replaces
  * reactionDiffusion_wrapper (reactionDiffusion.cu:568-577): in[] are read,
  * out[] written (never aliasing in[]), veltan[] (may be NULL) receives
  * rhs/dt (reactionDiffusion.cu:529-530,551-552).  Arrays: u,v for the 2V
  * model; u,v,w for the FK model.  solid may be NULL unless solidSwitch. */
 int yh_rd_step_dev(const yh_params* p, float* const out[3], const float* const in[3],
                    float* const veltan[3], const uint8_t* solid, void* stream);
+/* The same with the wrapper's mouse-stimulus arguments (stimLockMouse, point:
+ * reactionDiffusion.cu:54-61,134): while stim_on, cells within radius^2 < 400
+ * of (stim_x, stim_y) get the stimulus current 24.7.  Every branch runs with
+ * timeIntOrder 1; RK2/RK4 need stage fields and run through a plan
+ * (YH_EUNSUPPORTED here). */
+int yh_rd_step_dev_ex(const yh_params* p, float* const out[3], const float* const in[3],
+                      float* const veltan[3], const uint8_t* solid, int stim_on, int stim_x,
+                      int stim_y, void* stream);
 
 /* Spiral-tip record: the reference vec5dyn (typeDefinition.cuh:19-20). */
 typedef struct yh_tip { float x, y, vx, vy, t; } yh_tip;
@@ -168,6 +183,12 @@ int yh_read_tips(yh_plan* plan, yh_tip* out, int cap, int* n);
 int yh_clear_tips(yh_plan* plan);
 /* Device flag check: YH_ENONFINITE when any state value is NaN/Inf. */
 int yh_check_finite(yh_plan* plan, void* stream);
+
+/* Mouse stimulus of the plan's steps (stimLockMouse and point,
+ * reactionDiffusion.cu:54-61,134): on != 0 applies the current 24.7 within
+ * radius^2 < 400 cells of (x, y) in every following step until switched off.
+ * While on, the plan steps on the per-stage path. */
+int yh_set_stimulus(yh_plan* plan, int on, int x, int y);
 
 /* Plan options.  YH_OPT_KEEP_PREV (default 1): end every yh_step with a single
  * step so the previous state is kept for yh_track_tips; 0 lets yh_step use

@@ -15,7 +15,7 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* pi as the reference defines it (globalVariables.cuh via typeDefinition.cuh:152). */
+/* pi as the reference defines it (globalVariables.cuh:27). */
 #define ORC_PI (3.14159265359)
 
 void orc_derive(const yh_params* p, orc_coef* c) {

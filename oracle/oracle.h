@@ -88,6 +88,16 @@ void orc_sr_init(orc_sr_state* s, float tip0x, float tip0y);
 int orc_sr_step_d(const yh_params* p, const uint8_t* solid, double* const X[3],
                   orc_sr_state* s);
 
+/* The whole reactionDiffusion_kernel (reactionDiffusion.cu:36-561): Neumann
+ * or Dirichlet, square or masked, lap4, anisotropy, Euler/RK2/RK4 and the
+ * mouse stimulus (stim_on at the point (stim_x, stim_y)), race-free (each
+ * stage from the previous stage's field; lap4's neighbour currents from the
+ * neighbour's stage state).  FP64.  vt may be NULL.  Returns 0, or -1 for an
+ * unknown timeIntOrder. */
+int orc_rd_step_general_d(const yh_params* p, const uint8_t* solid, int stim_on, int stim_x,
+                          int stim_y, const double* const in[3], double* const out[3],
+                          double* const vt[3]);
+
 /* ------------------------------------------------------------------ APD */
 /* sAPD_kernel without stimulus area (spaceAPD.cu:278-374, the
  * stimulate==false branch), threshold 0.15. */

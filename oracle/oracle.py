@@ -61,6 +61,8 @@ def lib():
         h.orc_sr_step_d.restype = C.c_int
         h.orc_apd_update_d.argtypes = [C.c_int, C.c_double, C.c_long, P, P, P, P, P, P, P]
         h.orc_run_apd_d.argtypes = [PP, P, P, C.c_int, C.c_long, C.c_double, P, P, P, P, P]
+        h.orc_rd_step_general_d.argtypes = [PP, P, C.c_int, C.c_int, C.c_int, P, P, P]
+        h.orc_rd_step_general_d.restype = C.c_int
         h.orc_fk_dfield_step_d.argtypes = [PP, P, P, P, P]
         h.orc_fk_loss_d.argtypes = [PP, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
         h.orc_fk_loss_d.restype = C.c_double
@@ -113,6 +115,33 @@ def run(p, fields, nsteps, solid=None, dtype=np.float64, nthreads=None):
     fn = lib().orc_run_d if dtype == np.float64 else lib().orc_run_f
     fn(C.byref(p), _p(m), cst, int(nsteps), int(nthreads or os.cpu_count() or 1))
     return fs
+
+
+def rd_step_general(p, fields, solid=None, stim=None, veltan=False):
+    """One step of the whole reference kernel (every boundary, stencil and
+    time-integration branch), FP64.  stim: (x, y) of the mouse stimulus."""
+    nv = nvars(p)
+    fs = [np.ascontiguousarray(f, dtype=np.float64).reshape(-1) for f in fields[:nv]]
+    n = fs[0].size
+    outs = [np.empty(n) for _ in range(nv)]
+    vts = [np.empty(n) for _ in range(nv)] if veltan else None
+    cin = (C.c_void_p * 3)(*([_p(f) for f in fs] + [None] * (3 - nv)))
+    cout = (C.c_void_p * 3)(*([_p(f) for f in outs] + [None] * (3 - nv)))
+    cvt = (C.c_void_p * 3)(*([_p(f) for f in vts] + [None] * (3 - nv))) if veltan else None
+    m = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+    sx, sy = (int(stim[0]), int(stim[1])) if stim is not None else (0, 0)
+    rc = lib().orc_rd_step_general_d(C.byref(p), _p(m), int(stim is not None), sx, sy, cin, cout,
+                                     cvt)
+    if rc != 0:
+        raise ValueError("unsupported timeIntOrder")
+    return (outs, vts) if veltan else outs
+
+
+def run_general(p, fields, nsteps, solid=None, stim=None):
+    X = [np.asarray(f, np.float64) for f in fields]
+    for _ in range(nsteps):
+        X = rd_step_general(p, X, solid=solid, stim=stim)
+    return X
 
 
 def tips(p, g_past, g_present, time, cap=100000):

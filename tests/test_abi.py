@@ -66,10 +66,8 @@ def test_derive_matches_oracle(over):
 
 
 @pytest.mark.parametrize("over,code", [
-    (dict(neumannBC=0), T.YH_EUNSUPPORTED),
-    (dict(timeIntOrder=4), T.YH_EUNSUPPORTED),
-    (dict(lap4=4), T.YH_EUNSUPPORTED),
-    (dict(anisotropy=1), T.YH_EUNSUPPORTED),
+    (dict(timeIntOrder=3), T.YH_EINVAL),
+    (dict(timeIntOrder=0), T.YH_EINVAL),
     (dict(dt=2.0), T.YH_EUNSTABLE),
     (dict(nx=2), T.YH_EINVAL),
     (dict(model=7), T.YH_EINVAL),

@@ -31,6 +31,7 @@ PROTOTYPES = {
     "yh_init_random": (I, [I, I, C.c_uint64, C.c_float, P, P, P]),
     "yh_make_holes": (I, [I, I, I, D, D, C.c_uint64, P]),
     "yh_rd_step_dev": (I, [PARAMS_P, P, P, P, P, P]),
+    "yh_rd_step_dev_ex": (I, [PARAMS_P, P, P, P, P, I, I, I, P]),
     "yh_tip_detect_dev": (I, [PARAMS_P, P, P, D, P, P, P, I, P]),
     "yh_plan_create": (I, [PARAMS_P, C.POINTER(P)]),
     "yh_plan_destroy": (None, [P]),
@@ -48,6 +49,7 @@ PROTOTYPES = {
     "yh_clear_tips": (I, [P]),
     "yh_check_finite": (I, [P, P]),
     "yh_set_option": (I, [P, I, C.c_int64]),
+    "yh_set_stimulus": (I, [P, I, I, I]),
     "yh_apd_enable": (I, [P, I, D]),
     "yh_apd_get": (I, [P, P, P, P, P]),
     "yh_autotune": (I, [P, I, P, C.POINTER(YhKernelCfg), C.POINTER(C.c_float)]),

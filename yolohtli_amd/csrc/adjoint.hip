@@ -341,6 +341,7 @@ int yh_adjoint_create(const yh_params* p, int nsteps, int rec_every, int ckpt_ev
     return YH_EINVAL;
   if (p->model != YH_MODEL_3V_FK || !(p->heaviside_width > 0) || p->reduceSym)
     return YH_EINVAL;
+  if (!standard_branch(*p)) return YH_EUNSUPPORTED;
   *out = nullptr;
   yh_derived d;
   int rc = yh_params_derive(p, &d);

@@ -94,7 +94,8 @@ const char* yh_strerror(int code) {
   switch (code) {
     case YH_OK: return "ok";
     case YH_EINVAL: return "invalid argument";
-    case YH_EUNSUPPORTED: return "branch not built (Dirichlet, RK2/RK4, lap4, anisotropy)";
+    case YH_EUNSUPPORTED:
+      return "this path runs the standard branch only (Neumann, Euler, 5-point)";
     case YH_EHIP: return g_last_error.empty() ? "HIP runtime error" : g_last_error.c_str();
     case YH_ENOMEM: return "out of memory";
     case YH_EUNSTABLE: return "explicit Euler diffusion number rx+ry exceeds 0.5";
@@ -151,8 +152,9 @@ void yh_params_default(yh_params* p) {
   p->solidSwitch = 0;
   p->gateDiff = 1;
   p->reduceSym = 0;
-  // The reference defaults are RK4 + lap4 (saveFiles.cu:131-132); the built
-  // branch is Euler + 5-point.
+  // The reference defaults are RK4 + lap4 (saveFiles.cu:131-132); these
+  // defaults select Euler + 5-point, the branch of the temporally blocked
+  // kernels.  RK2/RK4, lap4, anisotropy and Dirichlet run on the general path.
   p->timeIntOrder = 1;
   p->lap4 = 0;
   p->anisotropy = 0;
@@ -170,7 +172,7 @@ int yh_params_derive(const yh_params* p, yh_derived* d) {
   if ((int64_t)p->nx * p->ny > (int64_t)1 << 31) return YH_EINVAL;
   if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->dt > 0)) return YH_EINVAL;
   if (p->model != YH_MODEL_2V_REF && p->model != YH_MODEL_3V_FK) return YH_EINVAL;
-  if (!p->neumannBC || p->timeIntOrder != 1 || p->lap4 != 0 || p->anisotropy) return YH_EUNSUPPORTED;
+  if (p->timeIntOrder != 1 && p->timeIntOrder != 2 && p->timeIntOrder != 4) return YH_EINVAL;
   if (p->model == YH_MODEL_3V_FK) {
     const yh_fk_params& f = p->fk;
     if (!(f.tau_d > 0 && f.tau_r > 0 && f.tau_si > 0 && f.tau_0 > 0 && f.tau_vp > 0 &&
@@ -252,6 +254,12 @@ int yh_make_holes(int nx, int ny, int nholes, double rmin, double rmax, uint64_t
 
 int yh_rd_step_dev(const yh_params* p, float* const out[3], const float* const in[3],
                    float* const veltan[3], const uint8_t* solid, void* stream) {
+  return yh_rd_step_dev_ex(p, out, in, veltan, solid, 0, 0, 0, stream);
+}
+
+int yh_rd_step_dev_ex(const yh_params* p, float* const out[3], const float* const in[3],
+                      float* const veltan[3], const uint8_t* solid, int stim_on, int stim_x,
+                      int stim_y, void* stream) {
   yh_derived d;
   int rc = yh_params_derive(p, &d);
   if (rc) return rc;
@@ -264,14 +272,22 @@ int yh_rd_step_dev(const yh_params* p, float* const out[3], const float* const i
       if ((const float*)out[a] == in[b]) return YH_EINVAL;
   }
   if (p->solidSwitch && !solid) return YH_EINVAL;
-  KParams k;
-  make_kparams(*p, d, &k);
   CFields ci{{in[0], in[1], nv > 2 ? in[2] : nullptr}};
   Fields co{{out[0], out[1], nv > 2 ? out[2] : nullptr}};
   Fields cv{{nullptr, nullptr, nullptr}};
   if (veltan) cv = Fields{{veltan[0], veltan[1], nv > 2 ? veltan[2] : nullptr}};
-  YH_HIP(launch_direct(p->model, k, ci, co, cv, p->solidSwitch ? solid : nullptr,
-                       (hipStream_t)stream));
+  if (standard_branch(*p) && !stim_on) {
+    KParams k;
+    make_kparams(*p, d, &k);
+    YH_HIP(launch_direct(p->model, k, ci, co, cv, p->solidSwitch ? solid : nullptr,
+                         (hipStream_t)stream));
+    return YH_OK;
+  }
+  // the stages of RK2/RK4 need scratch fields: a plan owns them
+  if (p->timeIntOrder != 1) return YH_EUNSUPPORTED;
+  const GParams g = make_gparams(*p, stim_on, stim_x, stim_y);
+  YH_HIP(launch_general_step(p->model, g, 1, ci, co, cv, p->solidSwitch ? solid : nullptr,
+                             nullptr, (hipStream_t)stream, nullptr));
   return YH_OK;
 }
 

@@ -45,6 +45,11 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
                            float* best_ms) {
   if (!pl || steps < 1) return YH_EINVAL;
   if (pl->p.solidSwitch && !pl->mask_set) return YH_ESTATE;
+  if (plan_general(pl)) {  // one kernel shape: nothing to choose
+    if (best) *best = pl->cfg;
+    if (best_ms) *best_ms = -1.0f;
+    return YH_OK;
+  }
   hipStream_t s = (hipStream_t)stream;
   const Key key{pl->p.nx, pl->p.ny, pl->p.model, pl->p.solidSwitch};
   {

@@ -276,6 +276,10 @@ int yh_batch_create(const yh_params* per_tissue, int ntissues, yh_batch** out) {
       batch_free(b);
       return YH_EINVAL;
     }
+    if (!standard_branch(p)) {
+      batch_free(b);
+      return YH_EUNSUPPORTED;
+    }
     yh_derived d;
     int rc = yh_params_derive(&p, &d);
     if (rc) {

@@ -46,6 +46,27 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
   ApdState apd = pl->apd;
   const ApdState* ap = pl->apd_on ? &apd : nullptr;
   int64_t done = pl->count;
+  if (plan_general(pl)) {
+    const int order = pl->p.timeIntOrder;
+    if (order > 1 && !pl->gscratch[0]) {
+      for (int q = 0; q < 9; ++q) YH_HIP(hipMalloc(&pl->gscratch[q], pl->n * sizeof(float)));
+    }
+    const GParams g = make_gparams(pl->p, pl->stim_on, pl->stim_x, pl->stim_y);
+    for (int q = 0; q < nsteps; ++q) {
+      CFields ci;
+      Fields co;
+      fields(pl->cur, &ci, &co);
+      apd.count0 = (int)done;
+      YH_HIP(launch_general_step(pl->p.model, g, order, ci, co, Fields{{nullptr, nullptr, nullptr}},
+                                 masked ? pl->solid_d : nullptr, order > 1 ? pl->gscratch : nullptr,
+                                 s, ap));
+      pl->cur ^= 1;
+      done += 1;
+    }
+    pl->have_prev = true;
+    pl->count += nsteps;
+    return YH_OK;
+  }
   const int tail = pl->keep_prev ? 1 : 0;
   if (c.variant == YH_KERNEL_TILED) {
     while (remaining > tail) {
@@ -92,6 +113,8 @@ static void plan_free(yh_plan* pl) {
   for (void* q : apd)
     if (q) (void)hipFree(q);
   if (pl->sr) sr_free(pl->sr);
+  for (float* q : pl->gscratch)
+    if (q) (void)hipFree(q);
   delete pl;
 }
 
@@ -257,6 +280,14 @@ int yh_clear_tips(yh_plan* pl) {
   if (!pl) return YH_EINVAL;
   YH_HIP(hipDeviceSynchronize());
   YH_HIP(hipMemset(pl->tipcount_d, 0, sizeof(int)));
+  return YH_OK;
+}
+
+int yh_set_stimulus(yh_plan* pl, int on, int x, int y) {
+  if (!pl) return YH_EINVAL;
+  pl->stim_on = on != 0;
+  pl->stim_x = x;
+  pl->stim_y = y;
   return YH_OK;
 }
 

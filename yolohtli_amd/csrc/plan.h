@@ -31,9 +31,14 @@ struct yh_plan {
   bool apd_on = false;
   yh::ApdState apd;  // device arrays (front, back, apd1, apd2, first) when apd_on
   yh_sr* sr = nullptr;
+  int stim_on = 0, stim_x = 0, stim_y = 0;  // mouse stimulus (yh_set_stimulus)
+  float* gscratch[9] = {};  // stage fields of the general path (RK2/RK4), lazily allocated
 };
 
 namespace yh {
+// Steps of this plan run on the general per-stage path (another branch, or
+// the mouse stimulus on).
+inline bool plan_general(const yh_plan* pl) { return !standard_branch(pl->p) || pl->stim_on; }
 // Resolve YH_KERNEL_AUTO to a concrete configuration for this plan.
 yh_kernel_cfg resolve_cfg(const yh_plan* pl);
 // Advance `nsteps` on the plan's buffers; the last step is a single step so

@@ -571,6 +571,7 @@ extern "C" {
 int yh_sr_begin(yh_plan* pl, float tip0x, float tip0y, void* stream) {
   if (!pl) return YH_EINVAL;
   if (!pl->p.reduceSym) return YH_ESTATE;  // dt must be the halved SR step (main.cu:148)
+  if (!standard_branch(pl->p)) return YH_EUNSUPPORTED;
   if (pl->p.solidSwitch && !pl->mask_set) return YH_ESTATE;
   hipStream_t s = (hipStream_t)stream;
   if (!pl->sr) {
@@ -617,6 +618,7 @@ int yh_sr_set_graph(yh_plan* pl, int on) {
 int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
   if (!pl || nsteps < 0) return YH_EINVAL;
   if (!pl->sr || !pl->sr->begun) return YH_ESTATE;
+  if (pl->stim_on) return YH_EUNSUPPORTED;  // the SR step runs the standard branch
   yh_sr* sr = pl->sr;
   hipStream_t s = (hipStream_t)stream;
   // history buffer for (c, phi) of each step

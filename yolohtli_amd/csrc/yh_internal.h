@@ -257,6 +257,30 @@ __device__ __forceinline__ void tip_cell(int i, int j, int nx, int ny, bool mask
 // sum_dir c_dir*(u_dir - u); 0xFF marks a hole.
 constexpr uint8_t kHole = 0xFF;
 
+// ------------------------------------------------- other stepper branches
+// Constants of the general path (general.hip): Dirichlet, lap4, anisotropy,
+// RK2/RK4, mouse stimulus, unfolded as the reference writes them.
+struct GParams {
+  int nx, ny, masked, neumann, lap4, aniso, vdiff;
+  float rx, ry, rxy, rbx, rby, q4, fx4, fy4, dt, tc, rscale, bval;  // q4 = qx4 + qy4
+  float mu, alpha, beta, gamma, delta, eps, theta;
+  float u_c, u_v, u_csi, kk, tau_d, tau_r, tau_si, tau_0, tau_vp, tau_v1m, tau_v2m, tau_wp,
+      tau_wm;
+  int stim_on, stim_x, stim_y;
+};
+GParams make_gparams(const yh_params& p, int stim_on, int stim_x, int stim_y);
+// The branches the fast kernels implement: Neumann, Euler, and the 5-point
+// Laplacian (with a mask, lap4 and the anisotropic term have no effect in the
+// reference, reactionDiffusion.cu:154-185).
+inline bool standard_branch(const yh_params& p) {
+  return p.neumannBC && p.timeIntOrder == 1 && (p.solidSwitch || (!p.lap4 && !p.anisotropy));
+}
+// One step of the general path: `order` stage launches (1, 2 or 4); scratch
+// holds 9 device arrays of n floats (stage increments x2, rhs) when order > 1.
+hipError_t launch_general_step(int model, const GParams& g, int order, CFields in, Fields out,
+                               Fields vt, const uint8_t* solid, float* const* scratch,
+                               hipStream_t s, const ApdState* apd);
+
 // ------------------------------------------------------------- launchers
 hipError_t launch_codes(const uint8_t* solid, uint8_t* codes, int nx, int ny, hipStream_t s);
 hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fields vt,

@@ -85,8 +85,10 @@ def _stream_handle(stream):
     return getattr(stream, "cuda_stream", stream)
 
 
-def reactionDiffusion_wrapper(p, gOut, gIn, velTan=None, solid=None, stream=None):
-    """One Euler step on caller-owned device fields (lists of tensors u,v[,w])."""
+def reactionDiffusion_wrapper(p, gOut, gIn, velTan=None, solid=None, stream=None,
+                              stimLockMouse=False, point=(0, 0)):
+    """One time step on caller-owned device fields (lists of tensors u,v[,w]);
+    stimLockMouse/point: the mouse stimulus of reactionDiffusion.cu:54-61."""
     nv = T.nvars(p)
     arr = C.c_void_p * 3
     out = arr(*([_dev_ptr(x) for x in gOut[:nv]] + [None] * (3 - nv)))
@@ -94,8 +96,9 @@ def reactionDiffusion_wrapper(p, gOut, gIn, velTan=None, solid=None, stream=None
     vt = None
     if velTan is not None:
         vt = arr(*([_dev_ptr(x) for x in velTan[:nv]] + [None] * (3 - nv)))
-    check(lib().yh_rd_step_dev(C.byref(p), out, inp, vt, _dev_ptr(solid),
-                               _stream_handle(stream)), "yh_rd_step_dev")
+    check(lib().yh_rd_step_dev_ex(C.byref(p), out, inp, vt, _dev_ptr(solid),
+                                  int(bool(stimLockMouse)), int(point[0]), int(point[1]),
+                                  _stream_handle(stream)), "yh_rd_step_dev_ex")
 
 
 def tip_wrapper(p, g_past, g_present, time, tips, keys, count, cap, stream=None):
@@ -232,6 +235,11 @@ class Plan:
         check(lib().yh_apd_get(self._h, _ptr(out["apd1"]), _ptr(out["apd2"]), _ptr(out["front"]),
                                _ptr(out["back"])), "yh_apd_get")
         return out
+
+    def set_stimulus(self, on, point=(0, 0)):
+        """Mouse stimulus (stimLockMouse, point) for the following steps."""
+        check(lib().yh_set_stimulus(self._h, int(bool(on)), int(point[0]), int(point[1])),
+              "yh_set_stimulus")
 
     def set_option(self, option, value):
         check(lib().yh_set_option(self._h, int(option), int(value)), "yh_set_option")
