@@ -141,6 +141,21 @@ int yh_tip_detect_dev(const yh_params* p, const float* g_past, const float* g_pr
                       double time, yh_tip* d_tips, int* d_keys, int* d_count, int cap,
                       void* stream);
 
+/* Contour points of countour_kernel (spaceAPD.cu:18-153) between two
+ * device fields, replacing countour_wrapper (:257-275), which zeroes the
+ * count first (so does this call).  mode 1: zero-crossings of field2 toward
+ * (i+1, j) / (i, j+1), interpolated linearly (the space-APD nodal lines, field2
+ * = sAPD); mode 2: cells where field1 < th1 and field2 crosses th2 (single
+ * contour); mode 3: the double contour of :97-138 (up to two points per cell).
+ * th1..th3 are the reference's conTh1..conTh3.  area (may be NULL =
+ * everywhere) is the reference's stimArea.  Each point is (x, y, time) in
+ * d_pts, its key 2*cell + sub in d_keys; entries past cap are dropped while
+ * *d_count still counts them.  At the last column / row the reference reads a
+ * wrapped or out-of-bounds neighbour; the cell's own value stands in here. */
+int yh_contour_dev(int nx, int ny, int mode, double th1, double th2, double th3,
+                   const float* field1, const float* field2, const uint8_t* area, float time,
+                   float* d_pts, int* d_keys, int* d_count, int cap, void* stream);
+
 /* ------------------------------------------------------------------ plans */
 typedef struct yh_plan yh_plan;
 
@@ -208,6 +223,14 @@ int yh_set_option(yh_plan* plan, int option, int64_t value);
 int yh_apd_enable(yh_plan* plan, int on, double threshold);
 /* Host copies of the per-cell fields; any pointer may be NULL. */
 int yh_apd_get(yh_plan* plan, float* apd1, float* apd2, float* front, float* back);
+
+/* Contours of the plan's current state (main.cu:968-977): modes 2 and 3
+ * between u and v, mode 1 on sAPD = sign(APD1 - APD2) of the plan's APD
+ * state (spaceAPD.cu:364; YH_ESTATE unless APD is enabled).  area: host mask
+ * (may be NULL).  pts receives up to cap points (x, y, t) sorted by (cell,
+ * sub); *n the number found (YH_EOVERFLOW when above cap).  Synchronous. */
+int yh_contour(yh_plan* plan, int mode, double th1, double th2, double th3, const uint8_t* area,
+               float* pts, int cap, int* n);
 
 /* Autotuner (no reference counterpart; the reference launches fixed 16x16
  * blocks, globalVariables.cuh:15-16): times the direct kernel and the tiled

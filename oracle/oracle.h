@@ -98,6 +98,15 @@ int orc_rd_step_general_d(const yh_params* p, const uint8_t* solid, int stim_on,
                           int stim_y, const double* const in[3], double* const out[3],
                           double* const vt[3]);
 
+/* countour_kernel (spaceAPD.cu:18-153): mode 1 zero-crossings of f2 with
+ * linear interpolation, mode 2 single and mode 3 double threshold contours
+ * of f2 where f1 < th1; area (may be NULL = everywhere) is stimArea.  out
+ * receives (x, y, t) per point, keys 2*cell+sub, in cell order.  Returns the
+ * number of points (beyond cap not written), -1 for an unknown mode. */
+int orc_contour_d(int nx, int ny, int mode, double th1, double th2, double th3,
+                  const double* f1, const double* f2, const uint8_t* area, float t,
+                  float* out, int* keys, int cap);
+
 /* ------------------------------------------------------------------ APD */
 /* sAPD_kernel without stimulus area (spaceAPD.cu:278-374, the
  * stimulate==false branch), threshold 0.15. */

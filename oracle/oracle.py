@@ -63,6 +63,9 @@ def lib():
         h.orc_run_apd_d.argtypes = [PP, P, P, C.c_int, C.c_long, C.c_double, P, P, P, P, P]
         h.orc_rd_step_general_d.argtypes = [PP, P, C.c_int, C.c_int, C.c_int, P, P, P]
         h.orc_rd_step_general_d.restype = C.c_int
+        h.orc_contour_d.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, P, P, P, C.c_float, P, P, C.c_int]
+        h.orc_contour_d.restype = C.c_int
         h.orc_fk_dfield_step_d.argtypes = [PP, P, P, P, P]
         h.orc_fk_loss_d.argtypes = [PP, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
         h.orc_fk_loss_d.restype = C.c_double
@@ -135,6 +138,20 @@ def rd_step_general(p, fields, solid=None, stim=None, veltan=False):
     if rc != 0:
         raise ValueError("unsupported timeIntOrder")
     return (outs, vts) if veltan else outs
+
+
+def contour(nx, ny, mode, th, f1, f2, area=None, t=0.0, cap=200000):
+    """(n, 3) contour points (x, y, t) in (cell, sub) order and their keys."""
+    a = np.ascontiguousarray(f1, dtype=np.float64).reshape(-1)
+    b = np.ascontiguousarray(f2, dtype=np.float64).reshape(-1)
+    m = None if area is None else np.ascontiguousarray(area, dtype=np.uint8).reshape(-1)
+    out = np.zeros((cap, 3), np.float32)
+    keys = np.zeros(cap, np.int32)
+    n = lib().orc_contour_d(nx, ny, mode, float(th[0]), float(th[1]), float(th[2]), _p(a), _p(b),
+                            _p(m), float(t), _p(out), _p(keys), cap)
+    if n < 0:
+        raise ValueError("unknown contour mode")
+    return out[:min(n, cap)].copy(), keys[:min(n, cap)].copy()
 
 
 def run_general(p, fields, nsteps, solid=None, stim=None):

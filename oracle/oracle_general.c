@@ -310,3 +310,72 @@ int orc_rd_step_general_d(const yh_params* p, const uint8_t* solid, int stim_on,
   free(buf);
   return 0;
 }
+
+/* ------------------------------------------------------------- contours */
+/* countour_kernel (spaceAPD.cu:18-153) for one cell, points appended in
+ * (cell, sub) order.  The reference reads field2 at (i+1, j) and (i, j+1)
+ * even on the last column / row (a wrapped or out-of-bounds element, used by
+ * mode 1's interpolation and mode 3's band test); here the cell's own value
+ * stands in, which makes that direction crossing-free as the zpm tests
+ * already assume.  mode 1's condition keeps the reference's precedence:
+ * zpmx < 0 || (zpmy < 0 && sc). */
+int orc_contour_d(int nx, int ny, int mode, double th1, double th2, double th3,
+                  const double* f1, const double* f2, const uint8_t* area, float t,
+                  float* out, int* keys, int cap) {
+  const double eps = 2.220446049250313e-16; /* MACHINE_EPS, globalVariables.cuh:26 */
+  int n = 0;
+#define ORC_PUSH(px, py, sub)                                        \
+  do {                                                               \
+    if (n < cap) {                                                   \
+      out[3 * n] = (float)(px);                                      \
+      out[3 * n + 1] = (float)(py);                                  \
+      out[3 * n + 2] = t;                                            \
+      keys[n] = 2 * c + (sub);                                       \
+    }                                                                \
+    ++n;                                                             \
+  } while (0)
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const int c = i + nx * j;
+      const int sc = area ? area[c] != 0 : 1;
+      const double f2c = f2[c];
+      const double f2x = i < nx - 1 ? f2[c + 1] : f2c;
+      const double f2y = j < ny - 1 ? f2[c + nx] : f2c;
+      if (mode == 1) {
+        const double v0 = f2c, v1x = f2x, v1y = f2y;
+        const double zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
+        const double zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
+        if ((zpmx < 0.0) || ((zpmy < 0.0) && sc)) {
+          const double ppx = fabs(v0 - v1x) > eps ? i + v0 / (v0 - v1x) : i;
+          const double ppy = fabs(v0 - v1y) > eps ? j + v0 / (v0 - v1y) : j;
+          ORC_PUSH(ppx, ppy, 0);
+        }
+      } else if (mode == 2) {
+        if ((f1[c] < th1) && sc) {
+          const double v0 = f2c - th2, v1x = f2x - th2, v1y = f2y - th2;
+          const double zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
+          const double zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
+          if ((zpmx < 0.0) || (zpmy < 0.0)) ORC_PUSH(i, j, 0);
+        }
+      } else if (mode == 3) {
+        const double V0 = f2c - th2, V1x = f2x - th2, V1y = f2y - th2;
+        if ((f1[c] < th1) && (fabs(V0) < (th3 + 0.1)) && (fabs(V1x) < (th3 + 0.1)) &&
+            (fabs(V1y) < (th3 + 0.1)) && sc) {
+          double v0 = V0 - th2, v1x = V1x - th2, v1y = V1y - th2;
+          double zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
+          double zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
+          if ((zpmx < 0.0) || (zpmy < 0.0)) ORC_PUSH(i, j, 0);
+          v0 = V0 + th3;
+          v1x = V1x + th3;
+          v1y = V1y + th3;
+          zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
+          zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
+          if ((zpmx < 0.0) || (zpmy < 0.0)) ORC_PUSH(i, j, 1);
+        }
+      } else {
+        return -1;
+      }
+    }
+#undef ORC_PUSH
+  return n;
+}

@@ -125,3 +125,59 @@ def test_general_branches_rejected_where_unsupported(torch_cuda):
     from yolohtli_amd.batch import Batch
     with pytest.raises(RuntimeError):
         Batch([T.grid_params(64, 64, model=T.YH_MODEL_3V_FK, solidSwitch=1, neumannBC=0)])
+
+
+@pytest.mark.parametrize("mode,th", [(1, (0.0, 0.0, 0.0)), (2, (0.5, 0.25, 0.0)),
+                                     (3, (0.5, 0.25, 0.375))])
+def test_contours_match_oracle(torch_cuda, mode, th):
+    """Contour points of two smooth random fields: the same cells (keys) and,
+    for mode 1, the same interpolated positions to FP32 rounding."""
+    torch = torch_cuda
+    nx, ny = 96, 80
+    rng = np.random.default_rng(40 + mode)
+    j, i = np.mgrid[0:ny, 0:nx]
+    f1 = (0.5 + 0.5 * np.sin(0.11 * i + 0.7) * np.cos(0.07 * j)).astype(np.float32)
+    f2 = (np.sin(0.09 * i - 0.3) * np.cos(0.13 * j + 0.2) * 0.6 + 0.3).astype(np.float32)
+    if mode == 1:
+        f2 = np.where(f2 > 0.3, 1.0, -1.0).astype(np.float32)
+    area = (rng.random((ny, nx)) > 0.1).astype(np.uint8)
+    cap = 2 * nx * ny
+    d1, d2 = torch.from_numpy(f1.reshape(-1)).cuda(), torch.from_numpy(f2.reshape(-1)).cuda()
+    da = torch.from_numpy(area.reshape(-1)).cuda()
+    pts = torch.zeros((cap, 3), dtype=torch.float32, device="cuda")
+    keys = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    cnt = torch.full((1,), 12345, dtype=torch.int32, device="cuda")
+    Y.solver.countour_wrapper(nx, ny, d1, d2, da, pts, keys, cnt, cap, 3.5, mode, th)
+    torch.cuda.synchronize()
+    n = int(cnt.item())
+    k = keys[:n].cpu().numpy()
+    order = np.argsort(k)
+    got, gk = pts[:n].cpu().numpy()[order], k[order]
+    want, wk = O.contour(nx, ny, mode, th, f1.astype(np.float64), f2.astype(np.float64),
+                         area=area, t=3.5)
+    assert n > 20
+    np.testing.assert_array_equal(gk, wk)
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
+
+
+def test_plan_contour_modes(torch_cuda):
+    """Plan contours of (u, v) and, with APD on, of sign(APD1 - APD2)."""
+    p = T.grid_params(64, 64)
+    f = Y.initGates(64, 64)
+    pl = Y.Plan(p)
+    pl.set_state(*f)
+    pl.step(300)
+    u, v = pl.get_state()
+    got = pl.contour(2, (0.5, 0.25, 0.0))
+    want, _ = O.contour(64, 64, 2, (0.5, 0.25, 0.0), u.astype(np.float64), v.astype(np.float64),
+                        t=O.derive(p).dt * 300)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-6)
+    with pytest.raises(RuntimeError):
+        pl.contour(1)  # APD not enabled
+    pl.apd_enable()
+    pl.step(3000)
+    a = pl.apd_get()
+    sapd = np.where(a["apd1"] - a["apd2"] > 0, 1.0, -1.0)
+    got = pl.contour(1)
+    want, _ = O.contour(64, 64, 1, (0, 0, 0), np.zeros(64 * 64), sapd, t=O.derive(p).dt * 3300)
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)

@@ -216,3 +216,50 @@ def test_time_integration_orders(model, y0, T_end):
         ratios = [errs[k] / errs[k + 1] for k in range(2)]
         print(model, order, errs, ratios)
         assert all(lo < r < hi for r in ratios), (order, errs, ratios)
+
+
+# ----------------------------------------------------------------- contours
+CX, CY = 40, 45
+
+
+def test_contour_mode1_interpolates_a_linear_crossing():
+    j, i = np.mgrid[0:CY, 0:CX].astype(np.float64)
+    f2 = i - 10.3
+    pts, keys = O.contour(CX, CY, 1, (0, 0, 0), np.zeros_like(f2), f2, t=2.5)
+    assert len(pts) == CY  # one crossing per row, between i = 10 and 11
+    np.testing.assert_allclose(pts[:, 0], 10.3, rtol=0, atol=1e-5)
+    np.testing.assert_array_equal(pts[:, 1], np.arange(CY))
+    assert np.all(pts[:, 2] == 2.5)
+    np.testing.assert_array_equal(keys, 2 * (10 + CX * np.arange(CY)))
+
+
+def test_contour_mode1_keeps_the_reference_precedence():
+    """zpmx < 0 || (zpmy < 0 && sc): stimArea masks only the y crossings."""
+    j, i = np.mgrid[0:CY, 0:CX].astype(np.float64)
+    off = np.zeros((CY, CX), np.uint8)
+    px, _ = O.contour(CX, CY, 1, (0, 0, 0), np.zeros_like(i), i - 10.3, area=off)
+    py, _ = O.contour(CX, CY, 1, (0, 0, 0), np.zeros_like(i), j - 10.3, area=off)
+    assert len(px) == CY and len(py) == 0
+
+
+def test_contour_mode2_single_threshold():
+    rng = np.random.default_rng(8)
+    j, i = np.mgrid[0:CY, 0:CX].astype(np.float64)
+    f1 = rng.random((CY, CX))
+    area = (rng.random((CY, CX)) > 0.2).astype(np.uint8)
+    pts, keys = O.contour(CX, CY, 2, (0.5, 7.0, 0.0), f1, j + 0.5, area=area)
+    want = [(x, 6) for x in range(CX) if f1[6, x] < 0.5 and area[6, x]]
+    assert [tuple(p[:2]) for p in pts] == want
+
+
+def test_contour_mode3_double_threshold():
+    rng = np.random.default_rng(9)
+    j, i = np.mgrid[0:CY, 0:CX].astype(np.float64)
+    f1 = rng.random((CY, CX))
+    f2 = 0.02 * j - 0.15
+    # th2 = 0.3, th3 = 0.4: band |f2 - th2| < 0.5; crossings of f2 - 2*th2
+    # (rows 37|38, sub 0) and of f2 - th2 + th3 (rows 2|3, sub 1)
+    pts, keys = O.contour(CX, CY, 3, (0.5, 0.3, 0.4), f1, f2)
+    want = sorted([2 * (x + CX * 37) for x in range(CX) if f1[37, x] < 0.5]
+                  + [2 * (x + CX * 2) + 1 for x in range(CX) if f1[2, x] < 0.5])
+    assert sorted(keys.tolist()) == want

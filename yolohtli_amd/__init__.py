@@ -9,11 +9,11 @@ from .abi_types import (  # noqa: F401
     YH_KERNEL_AUTO, YH_KERNEL_DIRECT, YH_KERNEL_TILED, YH_MODEL_2V_REF, YH_MODEL_3V_FK,
     default_params, grid_params, copy_params, nvars)
 from .solver import (  # noqa: F401
-    BASE_SEED, Plan, derive, initGates, init_random, make_holes, params,
+    BASE_SEED, Plan, countour_wrapper, derive, initGates, init_random, make_holes, params,
     reactionDiffusion_wrapper, seed, tip_wrapper)
 
 __all__ = [
     "Plan", "params", "derive", "seed", "initGates", "init_random", "make_holes",
-    "reactionDiffusion_wrapper", "tip_wrapper", "default_params", "grid_params",
+    "reactionDiffusion_wrapper", "tip_wrapper", "countour_wrapper", "default_params", "grid_params",
     "copy_params", "nvars", "BASE_SEED",
 ]

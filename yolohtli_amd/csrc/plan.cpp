@@ -115,6 +115,9 @@ static void plan_free(yh_plan* pl) {
   if (pl->sr) sr_free(pl->sr);
   for (float* q : pl->gscratch)
     if (q) (void)hipFree(q);
+  void* cont[] = {pl->cont_pts, pl->cont_keys, pl->cont_count, pl->cont_area, pl->sapd};
+  for (void* q : cont)
+    if (q) (void)hipFree(q);
   delete pl;
 }
 

@@ -33,6 +33,12 @@ struct yh_plan {
   yh_sr* sr = nullptr;
   int stim_on = 0, stim_x = 0, stim_y = 0;  // mouse stimulus (yh_set_stimulus)
   float* gscratch[9] = {};  // stage fields of the general path (RK2/RK4), lazily allocated
+  // contour search (yh_contour), lazily allocated
+  float* cont_pts = nullptr;
+  int* cont_keys = nullptr;
+  int* cont_count = nullptr;
+  uint8_t* cont_area = nullptr;
+  float* sapd = nullptr;
 };
 
 namespace yh {

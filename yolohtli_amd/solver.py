@@ -109,6 +109,18 @@ def tip_wrapper(p, g_past, g_present, time, tips, keys, count, cap, stream=None)
           "yh_tip_detect_dev")
 
 
+def countour_wrapper(nx, ny, field1, field2, area, pts, keys, count, cap, physicalTime, mode,
+                     thresholds=(0.8, 0.85, 0.7), stream=None):
+    """Contour points between two device fields (spaceAPD.cu:257-275, the
+    reference's spelling): pts (cap, 3) float32, keys (cap,) int32 and count
+    (1,) int32 device tensors; the count is zeroed first, as there."""
+    check(lib().yh_contour_dev(int(nx), int(ny), int(mode), *(float(x) for x in thresholds),
+                               _dev_ptr(field1), _dev_ptr(field2), _dev_ptr(area),
+                               float(physicalTime), _dev_ptr(pts), _dev_ptr(keys),
+                               _dev_ptr(count), int(cap), _stream_handle(stream)),
+          "yh_contour_dev")
+
+
 def tips_to_array(tips):
     """(n,5) float32 array x, y, vx, vy, t from a ctypes YhTip array or device bytes."""
     return np.ctypeslib.as_array(tips).view(np.float32).reshape(-1, 5) if len(tips) else \
@@ -240,6 +252,19 @@ class Plan:
         """Mouse stimulus (stimLockMouse, point) for the following steps."""
         check(lib().yh_set_stimulus(self._h, int(bool(on)), int(point[0]), int(point[1])),
               "yh_set_stimulus")
+
+    def contour(self, mode, thresholds=(0.8, 0.85, 0.7), area=None, cap=None):
+        """(n, 3) contour points (x, y, t) of the current state in cell order
+        (countour_kernel, spaceAPD.cu:18-153); thresholds = conTh1..3."""
+        cap = cap or 2 * self.n
+        out = np.empty((cap, 3), np.float32)
+        n = C.c_int()
+        a = None if area is None else np.ascontiguousarray(area, np.uint8).reshape(-1)
+        rc = lib().yh_contour(self._h, int(mode), *(float(x) for x in thresholds), _ptr(a),
+                              _ptr(out), cap, C.byref(n))
+        if rc not in (0, T.YH_EOVERFLOW):
+            check(rc, "yh_contour")
+        return out[:min(n.value, cap)].copy()
 
     def set_option(self, option, value):
         check(lib().yh_set_option(self._h, int(option), int(value)), "yh_set_option")
