@@ -97,6 +97,27 @@ def cpu_baseline(p, fields, mask, seconds):
             "fp64_GBps": round(f64["GBps"], 2)}
 
 
+def copy_bandwidth(stream, mib=256, reps=20):
+    """Device-to-device copy of `mib` MiB (read + write counted), HIP events on
+    the stream: the bandwidth a plain streaming kernel reaches on this GPU."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2 * (mib << 20) / (ms / 1e3) / 1e9
+
+
 def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell, launches=None):
     """Average duration of the dominant kernel: `launches` (default
     --kernel-launches) back-to-back launches of K fused steps each (no
@@ -127,8 +148,9 @@ def symmetry_reduction(a):
     """Symmetry-reduction configuration: 512^2 FK, cross-field start (u=1 for
     i<32, v=w=1 for j<ny/2, 0 above), plain steps until a tip exists (the
     count is recorded), then --sr-steps co-moving-frame steps at dt=0.01 ms,
-    each RD + tips + fused slices/inner products + 3x3 solve + 3 BFECC phases,
-    steady steps replayed from one captured hipGraph."""
+    each three launches (RD + tips; disk centre + slices with the inner
+    products; 3x3 solve + the three BFECC phases); steady steps replayed from
+    one captured hipGraph per state slot."""
     import torch
     import yolohtli_amd as Y
     from yolohtli_amd import abi_types as T
@@ -401,6 +423,8 @@ def main():
         f"({n * K / (t_launch_ms / 1e3) / 1e9:.1f} Gcell-steps/s inside one launch)")
 
     pl.close()
+    copy_gbps = copy_bandwidth(stream)
+    log(f"device copy of 256 MiB: {copy_gbps:.0f} GB/s ({copy_gbps / HBM_PEAK_GBPS:.3f} of peak)")
     control = None
     if not a.no_control and cfg[0] == T.YH_KERNEL_TILED and a.model == "fk":
         log("non-cache-resident control grid ...")
@@ -457,6 +481,7 @@ def main():
             "alg_bytes_per_launch": alg_bytes,
             "steps_per_launch": K,
             "equiv_GBps_per_step": round(bytes_per_cell * n * K / (t_launch_ms / 1e3) / 1e9, 1),
+            "copy_GBps": round(copy_gbps, 1),
         },
         "cpu_baseline": cpu,
         "batched": batch,

@@ -188,3 +188,57 @@ def test_check_finite_and_argument_errors(torch_cuda):
     pm.set_state(u, v)
     with pytest.raises(RuntimeError):
         pm.step(1)  # mask required first
+
+
+@pytest.mark.parametrize("shape", [(4, 4), (5, 7), (1000, 5), (6, 700), (129, 65), (257, 3 + 64 * 2)])
+@pytest.mark.parametrize("model", MODELS)
+def test_small_and_ragged_grids_match_oracle(torch_cuda, shape, model):
+    """Grids smaller than a tile, thinner than the halo, and ragged edges: the
+    tiled kernel (K clamped to the grid) against the FP64 oracle."""
+    nx, ny = shape
+    p = T.grid_params(nx, ny, model=model, solidSwitch=1)
+    rng = np.random.default_rng(nx * 1000 + ny)
+    mask = (rng.random(nx * ny) > 0.15).astype(np.uint8)
+    f = _fields(rng, p)
+    for cfg in ((T.YH_KERNEL_TILED, 8, 8, 8, 1), (T.YH_KERNEL_TILED, 3, 4, 16, 2),
+                (T.YH_KERNEL_DIRECT, 1, 0, 0, 1)):
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        pl.set_kernel(*cfg)
+        pl.step(30)
+        got = pl.get_state()
+        pl.close()
+        want = O.run(p, [x.astype(np.float64) for x in f], 30, solid=mask)
+        for a in range(len(f)):
+            d = np.abs(got[a] - want[a])
+            assert np.all(got[a][mask == 0] == 0)
+            if model == T.YH_MODEL_2V_REF:
+                assert d.max() < 1e-4, (cfg, a, d.max())
+            else:
+                assert np.mean(d > 1e-4) < 2e-3, (cfg, a, d.max())
+
+
+def test_4096_tiled_equals_one_step_kernel(torch_cuda):
+    """The largest bench grid (4096^2, 3200 holes): the headline tile shape and
+    the one-step kernel stay together over 48 steps; holes stay zero."""
+    n = 4096
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    mask = Y.make_holes(n, n, 3200, 3.0, 12.0, Y.seed(1, 1, 1))
+    f = Y.init_random(n, n, Y.seed(1, 1, 2), gate=1.0, nvars=3)
+    out = []
+    for cfg in ((T.YH_KERNEL_TILED, 8, 8, 8, 1), (T.YH_KERNEL_DIRECT, 1, 0, 0, 1)):
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        pl.set_kernel(*cfg)
+        pl.step(48)
+        assert pl.check_finite() == 0
+        out.append(pl.get_state())
+        pl.close()
+    hole = mask == 0
+    for a in range(3):
+        assert np.all(out[0][a][hole] == 0) and np.all(out[1][a][hole] == 0)
+        d = np.abs(out[0][a] - out[1][a])
+        print(a, d.max(), float(np.mean(d > 1e-5)))
+        assert np.mean(d > 1e-5) < 1e-4 and np.median(d) < 1e-7

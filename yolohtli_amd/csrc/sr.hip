@@ -1,25 +1,31 @@
 // sr.hip -- symmetry-reduction (co-moving frame) step on the device.
 //
-// One SR step follows display() (main.cu:894-953) in six launches:
-//   1 RD            X -> Xo                         (reactionDiffusion_kernel)
-//   2 k_sr_tips     g_past = Xo.u, g_present = X.u   (tip_wrapper, main.cu:900)
-//   3 k_sr_center   one workgroup: disk centre = last tip of this step, tipx0
-//                   at count 0; the step's tips move to the plan's log
-//   4 k_sr_slices   over the disk's bounding box: slices, template and velTan
-//                   in FP64, the 12 disk inner products as per-workgroup
-//                   partial sums (slice_kernel + trapz_kernel fused: no slice
-//                   arrays, no atomics)
-//   5 k_sr_solve    one workgroup: fixed-order reduction of the partials,
-//                   solve_matrix (symmetryReduction.cu:386-418), advection
-//                   parameters for Cxy_field (:45-56), phi += dt*c
-//   6 k_bf_fused    BFECC of Xo into X (advFDBFECC.cu:18-165): the three
-//                   dependent phases through an LDS tile with a 3-cell halo,
-//                   so no neighbour read races
-// At count 0 launches 4-5 run twice (main.cu:910-921).  Nothing leaves the
-// device; a steady step is captured once into a hipGraph and replayed.
-// Results cross workgroups only at kernel boundaries: on this 8-XCD part a
-// device-scope fence writes back the whole L2 of its XCD, so a
-// last-workgroup election in every workgroup costs far more than a launch.
+// One SR step follows display() (main.cu:894-953) in three launches:
+//   1 k_sr_rdtips   RD X -> Xo (reactionDiffusion_kernel) and the tips
+//                   between Xo.u and X.u (tip_wrapper, main.cu:900); a block
+//                   of the tip search also updates its one-cell north/east
+//                   halo through the same code path into LDS, so every
+//                   corner value is the one its own cell stores
+//   2 k_sr_slices   every workgroup picks the disk centre (last tip of this
+//                   step in (cell, root) order, tipx0 at count 0,
+//                   symmetryReduction.cu:95-107), then over the disk's
+//                   bounding box: slices, template and velTan in FP64, the
+//                   12 disk inner products as per-workgroup partial sums
+//                   (slice_kernel + trapz_kernel fused: no slice arrays)
+//   3 k_bf_fused    every workgroup reduces the partials in one fixed order
+//                   and solves solve_matrix (symmetryReduction.cu:386-418)
+//                   itself, so all agree bit for bit; then BFECC of Xo into X
+//                   (advFDBFECC.cu:18-165), the three dependent phases through
+//                   an LDS tile with a 3-cell halo.  Workgroup 0 commits the
+//                   step: c, phi += dt*c, centre, count, the tips to the log.
+// The SR state has two slots: a step reads slot count&1 and workgroup 0 of
+// launch 3 writes slot (count+1)&1, so no workgroup reads what another writes
+// in the same launch.  At count 0 a preliminary slices + solve pass runs
+// first (main.cu:910-921).  Nothing leaves the device; a steady step is
+// captured once per slot into a hipGraph and replayed.  Results cross
+// workgroups only at kernel boundaries: on this 8-XCD part a device-scope
+// fence writes back the whole L2 of its XCD, so a last-workgroup election in
+// every workgroup costs far more than a launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +34,7 @@
 #include <new>
 #include <vector>
 
+#include "direct.h"
 #include "plan.h"
 
 namespace yh {
@@ -50,6 +57,25 @@ struct SrGeom {
   double dtx, dty;  // dt/hx, dt/hy (advection Courant factors)
 };
 
+// Buffers of one step: the state slots, the step's tips, the partials, the
+// plan's tip log and the (c, phi) history.
+struct SrArgs {
+  const SrDev* st_in;
+  SrDev* st_out;
+  const double* partials;
+  int nparts;
+  int record;  // push (c, phi) before the solve into hist
+  double* hist;
+  long long histcap;
+  const yh_tip* step_tips;
+  const int* step_keys;
+  int* step_count;
+  yh_tip* log_tips;
+  int* log_keys;
+  int* log_count;
+  int logcap;
+};
+
 constexpr int kSrTipCap = 4096;
 constexpr int kSliceBX = 64, kSliceBY = 4;
 
@@ -58,91 +84,114 @@ __device__ __forceinline__ bool in_disk(const SrGeom& g, int i, int j, int cx, i
   return a * a + b * b < g.r2;
 }
 
-// ------------------------------------------------------------------- tips
-// Tips of one SR step over the cells the tip test can accept: the centred
-// disk's bounding box with a mask, the whole grid without (tip_cell).
-__global__ __launch_bounds__(256) void k_sr_tips(SrGeom g, int i0, int j0, int masked_disk,
-                                                 double Uth, const float* __restrict__ past,
-                                                 const float* __restrict__ present,
-                                                 const SrDev* st, yh_tip* tips, int* keys,
-                                                 int* count) {
-  const int i = i0 + blockIdx.x * 64 + threadIdx.x;
-  const int j = j0 + blockIdx.y * 4 + threadIdx.y;
-  if (i < 0 || i >= g.nx || j < 0 || j >= g.ny) return;
+// ------------------------------------------------------------- RD + tips
+// One thread per cell of a 64 x 4 block: the RD step of the cell; a block
+// holding cells of the tip search also updates its east column and north row
+// (69 halo cells, the same loop body, so each value equals what the cell's
+// own thread stores) into LDS, then runs the tip test between the new and
+// the old u (g_past = RD output, g_present = old state: main.cu:900,
+// tipTracker.cu:586).
+constexpr int kRtW = 65, kRtH = 5;  // the block's 64 x 4 cells + east column + north row
+
+template <int MODEL, bool MASKED>
+__global__ __launch_bounds__(256) void k_sr_rdtips(const KParams k, const SrGeom g, CFields X,
+                                                   Fields Xo, const uint8_t* __restrict__ codes,
+                                                   double Uth, const SrDev* __restrict__ st,
+                                                   yh_tip* tips, int* keys, int* count) {
+  using M = Model<MODEL>;
+  __shared__ float s_u[kRtH * kRtW];
+  const int nx = g.nx, ny = g.ny;
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 64 * ty;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = i < nx && j < ny;
+  const bool tip = valid && tip_inside(i, j, nx, ny, MASKED, g.r2);
+  const bool halo = __syncthreads_or(tip) && tid < kRtW + kRtH - 1;
+  // q = 0: the cell; q = 1: halo cell tid (north row, then east column)
+  const int nq = halo ? 2 : 1;
+#pragma nounroll
+  for (int q = 0; q < nq; ++q) {
+    const int li = q == 0 ? tx : (tid < kRtW ? tid : kRtW - 1);
+    const int lj = q == 0 ? ty : (tid < kRtW ? kRtH - 1 : tid - kRtW);
+    const int ii = i0 + li, jj = j0 + lj;
+    if (ii < nx && jj < ny) {
+      float xo[3], inc[3];
+      cell_update_codes<MODEL>(k, X.f, MASKED ? codes : nullptr, ii, jj, xo, inc);
+      if (q == 0) {
+#pragma unroll
+        for (int a = 0; a < M::NV; ++a) Xo.f[a][ii + nx * jj] = xo[a];
+      }
+      s_u[li + kRtW * lj] = xo[0];
+    }
+  }
+  __syncthreads();
+  if (!tip) return;
+  // corners past the last column/row are the cell itself (tipTracker.cu:58-63)
+  const bool ex = i < nx - 1, ey = j < ny - 1;
+  const int l0 = tx + kRtW * ty;
+  const int lx = ex ? l0 + 1 : l0, ly = ey ? l0 + kRtW : l0, lxy = (ex && ey) ? l0 + kRtW + 1 : l0;
+  const int c = i + nx * j;
+  const int sx = ex ? c + 1 : c, sy = ey ? c + nx : c, sxy = (ex && ey) ? c + nx + 1 : c;
+  const float* pr = X.f[0];
   // physicalTime = dt*count (main.cu:885)
   const float t = (float)(g.dt * (double)st->count);
-  tip_cell(i, j, g.nx, g.ny, masked_disk != 0, g.r2, Uth, past, present, t, tips, keys, count,
-           kSrTipCap);
+  tip_solve(i, j, nx, MASKED, Uth, pr[c], pr[sx], pr[sxy], pr[sy], s_u[l0], s_u[lx], s_u[lxy],
+            s_u[ly], t, tips, keys, count, kSrTipCap);
 }
 
 // ---------------------------------------------------------------- centre
-// One workgroup: the disk centre is the last tip of this step in (cell, root)
-// order, tipx0 at count 0 (symmetryReduction.cu:95-107), the previous centre
-// when there is no tip; then this step's tips move to the plan's log and the
-// step buffer is emptied.
-__global__ __launch_bounds__(256) void k_sr_center(SrGeom g, SrDev* st, const yh_tip* step_tips,
-                                                   const int* step_keys, int* step_count,
-                                                   yh_tip* log_tips, int* log_keys,
-                                                   int* log_count, int logcap) {
-  __shared__ int best_key[256];
-  __shared__ int best_slot[256];
-  __shared__ int base;
-  const int t = threadIdx.x;
-  const int found = *step_count;
-  const int n = min(found, kSrTipCap);
+// Disk centre of this step, the same in every wave that asks: tipx0 at
+// count 0 (symmetryReduction.cu:98-100), else the last tip of this step in
+// (cell, root) order (:102-103), else the previous centre.  Keys are unique,
+// so the wave maximum names one slot.
+__device__ __forceinline__ void sr_centre(const SrGeom& g, const SrDev* st, const yh_tip* tips,
+                                          const int* keys, const int* count, int& cx, int& cy) {
+  if (st->count == 0) {
+    cx = __float2int_rn(st->tip0x - (float)(g.nx / 2));
+    cy = __float2int_rn(st->tip0y - (float)(g.ny / 2));
+    return;
+  }
+  const int lane = __lane_id();
+  const int n = min(*count, kSrTipCap);
   int bk = -1, bs = -1;
-  for (int q = t; q < n; q += 256)
-    if (step_keys[q] > bk) {
-      bk = step_keys[q];
+  for (int q = lane; q < n; q += 64) {
+    const int kq = keys[q];
+    if (kq > bk) {
+      bk = kq;
       bs = q;
     }
-  best_key[t] = bk;
-  best_slot[t] = bs;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w && best_key[t + w] > best_key[t]) {
-      best_key[t] = best_key[t + w];
-      best_slot[t] = best_slot[t + w];
-    }
-    __syncthreads();
   }
-  if (t == 0) {
-    st->ntips = found;
-    if (st->count == 0) {
-      // tipx0_d at count 0 (symmetryReduction.cu:98-100)
-      st->cx = __float2int_rn(st->tip0x - (float)(g.nx / 2));
-      st->cy = __float2int_rn(st->tip0y - (float)(g.ny / 2));
-    } else if (best_slot[0] >= 0) {
-      // last tip of this step (symmetryReduction.cu:102-103); none: keep the centre
-      const yh_tip tp = step_tips[best_slot[0]];
-      st->cx = __float2int_rn(tp.x - (float)(g.nx / 2));
-      st->cy = __float2int_rn(tp.y - (float)(g.ny / 2));
-    }
-    base = atomicAdd(log_count, found);
-  }
-  __syncthreads();
-  for (int q = t; q < n; q += 256) {
-    const int slot = base + q;
-    if (slot < logcap) {
-      log_tips[slot] = step_tips[q];
-      log_keys[slot] = step_keys[q];
+  for (int off = 32; off > 0; off >>= 1) {
+    const int ok = __shfl_xor(bk, off, 64), os = __shfl_xor(bs, off, 64);
+    if (ok > bk) {
+      bk = ok;
+      bs = os;
     }
   }
-  __syncthreads();
-  if (t == 0) *step_count = 0;
+  if (bs >= 0) {
+    const yh_tip tp = tips[bs];
+    cx = __float2int_rn(tp.x - (float)(g.nx / 2));
+    cy = __float2int_rn(tp.y - (float)(g.ny / 2));
+  } else {
+    cx = st->cx;
+    cy = st->cy;
+  }
 }
 
 // ---------------------------------------------------------------- slices
-// Slices, template and velTan in FP64 for the cells of the disk around the
-// current centre (launched over the disk's bounding box, reach R), the 12
+// Slices, template and velTan in FP64 for the cells of the disk around this
+// step's centre (launched over the disk's bounding box, reach R), the 12
 // inner products as per-workgroup partial sums.
 template <int MODEL, bool MASKED>
 __global__ __launch_bounds__(kSliceBX* kSliceBY) void k_sr_slices(
     const KParams k, const SrGeom g, int R, CFields X, const uint8_t* __restrict__ codes,
-    const SrDev* __restrict__ st, double* __restrict__ partials) {
+    const SrDev* __restrict__ st, const yh_tip* __restrict__ step_tips,
+    const int* __restrict__ step_keys, const int* __restrict__ step_count,
+    double* __restrict__ partials) {
   using M = Model<MODEL>;
   constexpr int NV = M::NV;
-  const int cx = st->cx, cy = st->cy;
+  int cx, cy;
+  sr_centre(g, st, step_tips, step_keys, step_count, cx, cy);
   const int i = g.nx / 2 + cx - R + blockIdx.x * kSliceBX + threadIdx.x;
   const int j = g.ny / 2 + cy - R + blockIdx.y * kSliceBY + threadIdx.y;
   double acc[12];
@@ -221,43 +270,36 @@ __global__ __launch_bounds__(kSliceBX* kSliceBY) void k_sr_slices(
   if (tid < 12) {
     double s = 0.0;
     for (int w = 0; w < kSliceBX * kSliceBY / 64; ++w) s += red[w][tid];
-    partials[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * 12 + tid] = s;
+    partials[(size_t)tid * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = s;
   }
 }
 
 // ----------------------------------------------------------------- solve
-// One workgroup of 12 waves: wave q sums column q of the partials in a fixed
-// order (lane-strided, then a shuffle tree), so the integrals do not depend
-// on scheduling; then solve_matrix (symmetryReduction.cu:386-418), the
-// advection parameters of Cxy_field (:45-56) and phi += dt*c.
-__global__ __launch_bounds__(768) void k_sr_solve(SrGeom g, SrDev* st, const double* partials,
-                                                  int nparts, int pre, int record, double* hist,
-                                                  long long histcap) {
-  __shared__ double Is[12];
-  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// Column q of the partials (stored column-major, [12][nparts]) summed by one
+// wave in a fixed order (lane-strided, then a shuffle tree), so the integrals do not depend on scheduling and
+// every wave that sums a column gets the same bits.  The total is in lane 0.
+__device__ __forceinline__ double sr_column_sum(const double* partials, int nparts, int q) {
+  const int lane = __lane_id();
   double s = 0.0;
-  for (int b = lane; b < nparts; b += 64) s += partials[(size_t)b * 12 + q];
+  const double* col = partials + (size_t)q * nparts;
+  for (int b = lane; b < nparts; b += 64) s += col[b];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) Is[q] = s;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
+  return s;
+}
+
+// One thread: the integrals sums*hx*hy, solve_matrix
+// (symmetryReduction.cu:386-418), the advection parameters of Cxy_field
+// (:45-56) with the new c and the current phi, and, when advance, the Euler
+// update phi += dt*c (reduceSymStart stays set, main.cu:935-941) and the
+// step count.
+__device__ void sr_solve(const SrGeom& g, const double* sums, SrDev& o, bool advance) {
   double I[12];
   for (int p = 0; p < 12; ++p) {
-    I[p] = Is[p] * (g.hx * g.hy);
-    st->I[p] = I[p];
+    I[p] = sums[p] * (g.hx * g.hy);
+    o.I[p] = I[p];
   }
-  if (record && hist && st->count < histcap) {
-    // clist/philist push (main.cu:902-903): c and phi before this step's
-    // first solve (at count 0 that is the preliminary solve, main.cu:910-914)
-    double* h = hist + (size_t)st->count * 6;
-    for (int p = 0; p < 3; ++p) {
-      h[p] = st->c[p];
-      h[3 + p] = st->phi[p];
-    }
-  }
-  // solve_matrix, symmetryReduction.cu:386-418
   double sp, cp;
-  sincos(st->phi[2], &sp, &cp);
+  sincos(o.phi[2], &sp, &cp);
   const double a1 = I[0] * cp + I[1] * sp, a2 = I[1] * cp - I[0] * sp, a3 = I[2];
   const double b1 = I[3] * cp + I[4] * sp, b2 = I[4] * cp - I[3] * sp, b3 = I[5];
   const double C1 = I[6] * cp + I[7] * sp, C2 = I[7] * cp - I[6] * sp, C3 = I[8];
@@ -268,21 +310,47 @@ __global__ __launch_bounds__(768) void k_sr_solve(SrGeom g, SrDev* st, const dou
   const double x3 = d3pp / c3pp;
   const double x2 = (d2p - b3p * x3) / b2p;
   const double x1 = (d1 - a2 * x2 - a3 * x3) / a1;
-  if (!(isfinite(x1) && isfinite(x2) && isfinite(x3))) st->status |= 1;
-  st->c[0] = x1;
-  st->c[1] = x2;
-  st->c[2] = x3;
-  // Cxy_field_kernel (symmetryReduction.cu:45-56) with the new c and the
-  // current phi (the angle of the solve above)
-  st->aphi = st->phi[2];
-  st->ac2 = x3;
-  st->axc = -x1 * cp + x2 * sp;
-  st->ayc = -x1 * sp - x2 * cp;
-  if (!pre) {
-    // phi += dt*c (Euler: reduceSymStart stays set, main.cu:935-941)
-    for (int p = 0; p < 3; ++p) st->phi[p] = st->phi[p] + st->c[p] * g.dt;
-    st->count += 1;
+  if (!(isfinite(x1) && isfinite(x2) && isfinite(x3))) o.status |= 1;
+  o.c[0] = x1;
+  o.c[1] = x2;
+  o.c[2] = x3;
+  o.aphi = o.phi[2];
+  o.ac2 = x3;
+  o.axc = -x1 * cp + x2 * sp;
+  o.ayc = -x1 * sp - x2 * cp;
+  if (advance) {
+    for (int p = 0; p < 3; ++p) o.phi[p] = o.phi[p] + o.c[p] * g.dt;
+    o.count += 1;
   }
+}
+
+// (c, phi) of the state before this step's first solve: the clist/philist
+// push (main.cu:902-903); at count 0 that precedes the preliminary solve
+// (main.cu:910-914).
+__device__ __forceinline__ void sr_record(const SrDev& o, double* hist, long long histcap) {
+  if (!hist || o.count >= histcap) return;
+  double* h = hist + (size_t)o.count * 6;
+  for (int p = 0; p < 3; ++p) {
+    h[p] = o.c[p];
+    h[3 + p] = o.phi[p];
+  }
+}
+
+// The preliminary solve of count 0 (main.cu:910-921), in place: one
+// workgroup of 12 waves, wave q sums column q.
+__global__ __launch_bounds__(768) void k_sr_solve_pre(SrGeom g, SrDev* st, const double* partials,
+                                                      int nparts, double* hist,
+                                                      long long histcap) {
+  __shared__ double Is[12];
+  const int q = threadIdx.x >> 6;
+  const double s = sr_column_sum(partials, nparts, q);
+  if (__lane_id() == 0) Is[q] = s;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  SrDev o = *st;
+  sr_record(o, hist, histcap);
+  sr_solve(g, Is, o, false);
+  *st = o;
 }
 
 // ----------------------------------------------------------------- BFECC
@@ -341,7 +409,7 @@ __device__ __forceinline__ BfIdx bf_index(uint32_t f, int c, int stride) {
 // advFDBFECC.cu:30-31) and neighbour flags are computed once.  Cells outside
 // the domain are never read: the boundary rules substitute an inside
 // neighbour.
-constexpr int kBfTX = 64, kBfTY = 8, kBfHalo = 3;
+constexpr int kBfTX = 64, kBfTY = 8, kBfHalo = 3, kBfThreads = 512;
 constexpr int kBfLW = kBfTX + 2 * kBfHalo, kBfLH = kBfTY + 2 * kBfHalo;
 constexpr int kBfLN = kBfLW * kBfLH;
 
@@ -353,10 +421,11 @@ __device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[kBfLN],
                                          const float* s_rx, const float* s_ry,
                                          const uint32_t* s_fl, Fields out, int i0, int j0,
                                          int nx) {
-  constexpr int W = kBfLW - 2 * PH, H = kBfLH - 2 * PH, N = W * H, IT = (N + 255) / 256;
+  constexpr int W = kBfLW - 2 * PH, H = kBfLH - 2 * PH, N = W * H;
+  constexpr int IT = (N + kBfThreads - 1) / kBfThreads;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    const int l = tid + it * 256;
+    const int l = tid + it * kBfThreads;
     if (l < N) {
       const int L = (PH + l % W) + kBfLW * (PH + l / W);
       const uint32_t fl = s_fl[L];
@@ -393,29 +462,90 @@ __device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[kBfLN],
 }
 
 template <bool MASKED, int NV>
-__global__ __launch_bounds__(256) void k_bf_fused(SrGeom g, const SrDev* st, CFields in,
-                                                  Fields out, const uint8_t* __restrict__ solid) {
+__global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CFields in, Fields out,
+                                                  const uint8_t* __restrict__ solid) {
   __shared__ float s_q[NV][kBfLN];  // the state advected, then the corrected field ue
   __shared__ float s_f[NV][kBfLN];  // forward step uf
   __shared__ float s_rx[kBfLN], s_ry[kBfLN];
   __shared__ uint32_t s_fl[kBfLN];  // bf_flags | px << 5 | py << 6 | inside << 7
+  __shared__ double s_I[12];
+  __shared__ double s_adv[3];  // ac2, axc, ayc
+  __shared__ int s_cx, s_cy, s_found, s_base;
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
   const int nx = g.nx, ny = g.ny;
   const int i0 = blockIdx.x * kBfTX - kBfHalo, j0 = blockIdx.y * kBfTY - kBfHalo;
-  const double ac2 = st->ac2, axc = st->axc, ayc = st->ayc;
-  constexpr int IT = (kBfLN + 255) / 256;
+  const bool commit = blockIdx.x == 0 && blockIdx.y == 0;
+  // 1 the integrals (wave w sums columns w, w+8); the state tile and the
+  //   neighbour flags, which do not depend on the solve; in workgroup 0 the
+  //   centre and the tip log slots
+  for (int q = wv; q < 12; q += kBfThreads / 64) {
+    const double v = sr_column_sum(sa.partials, sa.nparts, q);
+    if (lane == 0) s_I[q] = v;
+  }
+  constexpr int IT = (kBfLN + kBfThreads - 1) / kBfThreads;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    const int l = tid + it * 256;
+    const int l = tid + it * kBfThreads;
     if (l < kBfLN) {
       const int gi = i0 + l % kBfLW, gj = j0 + l / kBfLW;
       const bool ok = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
 #pragma unroll
       for (int a = 0; a < NV; ++a) s_q[a][l] = ok ? in.f[a][gi + (size_t)nx * gj] : 0.0f;
-      uint32_t fl = 0;
+      s_fl[l] = ok ? (bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u) : 0u;
+    }
+  }
+  if (commit && wv == 0) {
+    int cx, cy;
+    sr_centre(g, sa.st_in, sa.step_tips, sa.step_keys, sa.step_count, cx, cy);
+    if (lane == 0) {
+      const int found = *sa.step_count;
+      s_cx = cx;
+      s_cy = cy;
+      s_found = found;
+      s_base = atomicAdd(sa.log_count, found);
+    }
+  }
+  __syncthreads();
+  // 2 the solve, in every workgroup from the same sums; workgroup 0 writes
+  //   the next state slot and moves this step's tips to the log
+  if (tid == 0) {
+    SrDev o = *sa.st_in;
+    if (commit && sa.record) sr_record(o, sa.hist, sa.histcap);
+    sr_solve(g, s_I, o, true);
+    s_adv[0] = o.ac2;
+    s_adv[1] = o.axc;
+    s_adv[2] = o.ayc;
+    if (commit) {
+      o.cx = s_cx;
+      o.cy = s_cy;
+      o.ntips = s_found;
+      *sa.st_out = o;
+    }
+  }
+  if (commit) {
+    const int n = min(s_found, kSrTipCap);
+    for (int q = tid; q < n; q += kBfThreads) {
+      const int slot = s_base + q;
+      if (slot < sa.logcap) {
+        sa.log_tips[slot] = sa.step_tips[q];
+        sa.log_keys[slot] = sa.step_keys[q];
+      }
+    }
+  }
+  __syncthreads();
+  if (commit && tid == 0) *sa.step_count = 0;
+  // 3 advection coefficients of each tile cell (Cxy_field_kernel,
+  //   symmetryReduction.cu:45-56, negated as at advFDBFECC.cu:30-31)
+  const double ac2 = s_adv[0], axc = s_adv[1], ayc = s_adv[2];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int l = tid + it * kBfThreads;
+    if (l < kBfLN) {
+      uint32_t fl = s_fl[l];
       float rx = 0.0f, ry = 0.0f;
-      if (ok) {
-        fl = bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u;
+      if (fl & 128u) {
+        const int gi = i0 + l % kBfLW, gj = j0 + l / kBfLW;
         double ax = g.hy * (double)gj * ac2 + axc;
         double ay = -g.hx * (double)gi * ac2 + ayc;
         if (MASKED && !(fl & 1u)) ax = ay = 0.0;  // Cxy_field_kernel :50-51
@@ -442,7 +572,8 @@ __global__ __launch_bounds__(256) void k_bf_fused(SrGeom g, const SrDev* st, CFi
 }  // namespace yh
 
 struct yh_sr {
-  yh::SrDev* st = nullptr;
+  yh::SrDev* st = nullptr;  // two slots: a step reads st[count & 1], writes the other
+  long long count = 0;      // steps taken since yh_sr_begin (the host's copy)
   double* partials = nullptr;
   int nparts = 0;
   yh_tip* step_tips = nullptr;
@@ -453,18 +584,27 @@ struct yh_sr {
   long long histcap = 0;
   bool begun = false;
   bool use_graph = true;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr};  // one steady step per slot
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int graph_cur[2] = {-1, -1};
   hipStream_t cap_stream = nullptr;
-  int graph_cur = -1;
 };
 
 namespace yh {
 
+static void sr_drop_graphs(yh_sr* sr) {
+  for (int q = 0; q < 2; ++q) {
+    if (sr->exec[q]) (void)hipGraphExecDestroy(sr->exec[q]);
+    if (sr->graph[q]) (void)hipGraphDestroy(sr->graph[q]);
+    sr->exec[q] = nullptr;
+    sr->graph[q] = nullptr;
+    sr->graph_cur[q] = -1;
+  }
+}
+
 void sr_free(yh_sr* sr) {
   if (!sr) return;
-  if (sr->exec) (void)hipGraphExecDestroy(sr->exec);
-  if (sr->graph) (void)hipGraphDestroy(sr->graph);
+  sr_drop_graphs(sr);
   if (sr->cap_stream) (void)hipStreamDestroy(sr->cap_stream);
   if (sr->st) (void)hipFree(sr->st);
   if (sr->partials) (void)hipFree(sr->partials);
@@ -502,10 +642,19 @@ static int disk_reach(int r2) {
   return R;
 }
 
-// Enqueue one SR step (first: the count-0 double solve) on stream s:
-//   RD X -> Xo; tips; centre; (slices, solve) once, twice at count 0;
-//   BFECC of Xo back into X.  Six launches in a steady step.
-static int sr_enqueue(yh_plan* pl, bool first, hipStream_t s) {
+template <int MODEL, bool MASKED>
+static void sr_launch_slices(const yh_plan* pl, const SrGeom& g, int R, CFields X, const SrDev* st,
+                             dim3 sg, dim3 sb, hipStream_t s) {
+  const yh_sr* sr = pl->sr;
+  hipLaunchKernelGGL((k_sr_slices<MODEL, MASKED>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, st,
+                     (const yh_tip*)sr->step_tips, (const int*)sr->step_keys,
+                     (const int*)sr->step_count, sr->partials);
+}
+
+// Enqueue one SR step on stream s, reading state slot `slot` (first: the
+// count-0 step with its preliminary slices + solve):
+//   RD + tips; slices; BFECC with the solve.  Three launches in a steady step.
+static int sr_enqueue(yh_plan* pl, bool first, int slot, hipStream_t s) {
   yh_sr* sr = pl->sr;
   const SrGeom g = sr_geom(pl);
   const bool masked = pl->p.solidSwitch != 0;
@@ -514,49 +663,65 @@ static int sr_enqueue(yh_plan* pl, bool first, hipStream_t s) {
   Fields Xo{{pl->buf[b ^ 1][0], pl->buf[b ^ 1][1], pl->buf[b ^ 1][2]}};
   CFields Xoc{{Xo.f[0], Xo.f[1], Xo.f[2]}};
   Fields Xw{{pl->buf[b][0], pl->buf[b][1], pl->buf[b][2]}};
+  SrDev* st_in = sr->st + slot;
+  SrDev* st_out = sr->st + (slot ^ 1);
   const int R = sr->reach, box = 2 * R + 1;
-  // 1 RD
-  YH_HIP(launch_direct_codes(pl->p.model, pl->k, X, Xo, masked ? pl->codes_d : nullptr, s));
-  // 2 tips (g_past = RD output, g_present = old state: main.cu:900, tipTracker.cu:586);
-  //   with a mask only the centred disk's box can hold tips
+  const int model = pl->p.model;
+  // 1 RD + tips
   {
-    const int i0 = masked ? g.nx / 2 - R : 0, j0 = masked ? g.ny / 2 - R : 0;
-    const int w = masked ? box : g.nx, h = masked ? box : g.ny;
-    hipLaunchKernelGGL(k_sr_tips, dim3((w + 63) / 64, (h + 3) / 4), dim3(64, 4), 0, s, g, i0, j0,
-                       (int)masked, pl->p.Uth, (const float*)Xo.f[0], X.f[0], sr->st,
-                       sr->step_tips, sr->step_keys, sr->step_count);
-    YH_HIP(hipGetLastError());
-  }
-  // 3 centre, tips to the log
-  hipLaunchKernelGGL(k_sr_center, dim3(1), dim3(256), 0, s, g, sr->st, sr->step_tips,
-                     sr->step_keys, sr->step_count, pl->tips_d, pl->keys_d, pl->tipcount_d,
-                     pl->tipcap);
-  YH_HIP(hipGetLastError());
-  // 4 slices + inner products over the moving disk's box, 5 solve
-  dim3 sb(kSliceBX, kSliceBY), sg((box + kSliceBX - 1) / kSliceBX, (box + kSliceBY - 1) / kSliceBY);
-  for (int pass = first ? 0 : 1; pass < 2; ++pass) {
-    const int pre = pass == 0;
-    const int record = pass == (first ? 0 : 1);
-    if (pl->p.model == M2V) {
-      if (masked) hipLaunchKernelGGL((k_sr_slices<M2V, true>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
-      else hipLaunchKernelGGL((k_sr_slices<M2V, false>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
+    dim3 rb(64, 4), rg((g.nx + 63) / 64, (g.ny + 3) / 4);
+    const uint8_t* codes = pl->codes_d;
+    const double Uth = pl->p.Uth;
+    if (model == M2V) {
+      if (masked) hipLaunchKernelGGL((k_sr_rdtips<M2V, true>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
+      else hipLaunchKernelGGL((k_sr_rdtips<M2V, false>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
     } else {
-      if (masked) hipLaunchKernelGGL((k_sr_slices<MFK, true>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
-      else hipLaunchKernelGGL((k_sr_slices<MFK, false>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, sr->st, sr->partials);
+      if (masked) hipLaunchKernelGGL((k_sr_rdtips<MFK, true>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
+      else hipLaunchKernelGGL((k_sr_rdtips<MFK, false>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
     }
     YH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_sr_solve, dim3(1), dim3(768), 0, s, g, sr->st, sr->partials,
-                       sr->nparts, pre, record, sr->hist, sr->histcap);
-    YH_HIP(hipGetLastError());
   }
-  // 6 BFECC of Xo back into X, the three phases in one launch
+  // 2 slices + inner products over the moving disk's box (at count 0 first
+  //   the preliminary pass and its solve, in place)
+  dim3 sb(kSliceBX, kSliceBY), sg((box + kSliceBX - 1) / kSliceBX, (box + kSliceBY - 1) / kSliceBY);
+  for (int pass = first ? 0 : 1; pass < 2; ++pass) {
+    if (model == M2V) {
+      if (masked) sr_launch_slices<M2V, true>(pl, g, R, X, st_in, sg, sb, s);
+      else sr_launch_slices<M2V, false>(pl, g, R, X, st_in, sg, sb, s);
+    } else {
+      if (masked) sr_launch_slices<MFK, true>(pl, g, R, X, st_in, sg, sb, s);
+      else sr_launch_slices<MFK, false>(pl, g, R, X, st_in, sg, sb, s);
+    }
+    YH_HIP(hipGetLastError());
+    if (pass == 0) {
+      hipLaunchKernelGGL(k_sr_solve_pre, dim3(1), dim3(768), 0, s, g, st_in,
+                         (const double*)sr->partials, sr->nparts, sr->hist, sr->histcap);
+      YH_HIP(hipGetLastError());
+    }
+  }
+  // 3 solve + BFECC of Xo back into X, the three phases in one launch
+  SrArgs sa;
+  sa.st_in = st_in;
+  sa.st_out = st_out;
+  sa.partials = sr->partials;
+  sa.nparts = sr->nparts;
+  sa.record = first ? 0 : 1;  // at count 0 the preliminary solve recorded
+  sa.hist = sr->hist;
+  sa.histcap = sr->histcap;
+  sa.step_tips = sr->step_tips;
+  sa.step_keys = sr->step_keys;
+  sa.step_count = sr->step_count;
+  sa.log_tips = pl->tips_d;
+  sa.log_keys = pl->keys_d;
+  sa.log_count = pl->tipcount_d;
+  sa.logcap = pl->tipcap;
   dim3 bg((g.nx + kBfTX - 1) / kBfTX, (g.ny + kBfTY - 1) / kBfTY);
   if (g.nv == 3) {
-    if (masked) hipLaunchKernelGGL((k_bf_fused<true, 3>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
-    else hipLaunchKernelGGL((k_bf_fused<false, 3>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
+    if (masked) hipLaunchKernelGGL((k_bf_fused<true, 3>), bg, dim3(kBfThreads), 0, s, g, sa, Xoc, Xw, pl->solid_d);
+    else hipLaunchKernelGGL((k_bf_fused<false, 3>), bg, dim3(kBfThreads), 0, s, g, sa, Xoc, Xw, pl->solid_d);
   } else {
-    if (masked) hipLaunchKernelGGL((k_bf_fused<true, 2>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
-    else hipLaunchKernelGGL((k_bf_fused<false, 2>), bg, dim3(256), 0, s, g, sr->st, Xoc, Xw, pl->solid_d);
+    if (masked) hipLaunchKernelGGL((k_bf_fused<true, 2>), bg, dim3(kBfThreads), 0, s, g, sa, Xoc, Xw, pl->solid_d);
+    else hipLaunchKernelGGL((k_bf_fused<false, 2>), bg, dim3(kBfThreads), 0, s, g, sa, Xoc, Xw, pl->solid_d);
   }
   YH_HIP(hipGetLastError());
   return YH_OK;
@@ -582,29 +747,23 @@ int yh_sr_begin(yh_plan* pl, float tip0x, float tip0y, void* stream) {
     const int box = 2 * sr->reach + 1;
     const int gx = (box + kSliceBX - 1) / kSliceBX, gy = (box + kSliceBY - 1) / kSliceBY;
     sr->nparts = gx * gy;
-    YH_HIP(hipMalloc(&sr->st, sizeof(SrDev)));
+    YH_HIP(hipMalloc(&sr->st, 2 * sizeof(SrDev)));
     YH_HIP(hipMalloc(&sr->partials, sizeof(double) * 12 * sr->nparts));
     YH_HIP(hipMalloc(&sr->step_tips, sizeof(yh_tip) * kSrTipCap));
     YH_HIP(hipMalloc(&sr->step_keys, sizeof(int) * kSrTipCap));
     YH_HIP(hipMalloc(&sr->step_count, sizeof(int)));
   }
   yh_sr* sr = pl->sr;
-  SrDev h;
-  std::memset(&h, 0, sizeof h);
-  h.tip0x = tip0x;
-  h.tip0y = tip0y;
-  YH_HIP(hipMemcpyAsync(sr->st, &h, sizeof h, hipMemcpyHostToDevice, s));
+  SrDev h[2];
+  std::memset(h, 0, sizeof h);
+  h[0].tip0x = h[1].tip0x = tip0x;
+  h[0].tip0y = h[1].tip0y = tip0y;
+  YH_HIP(hipMemcpyAsync(sr->st, h, sizeof h, hipMemcpyHostToDevice, s));
   YH_HIP(hipMemsetAsync(sr->step_count, 0, sizeof(int), s));
   YH_HIP(hipStreamSynchronize(s));
+  sr->count = 0;
   sr->begun = true;
-  if (sr->exec) {
-    (void)hipGraphExecDestroy(sr->exec);
-    sr->exec = nullptr;
-  }
-  if (sr->graph) {
-    (void)hipGraphDestroy(sr->graph);
-    sr->graph = nullptr;
-  }
+  sr_drop_graphs(sr);
   pl->have_prev = false;
   return YH_OK;
 }
@@ -621,14 +780,8 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
   if (pl->stim_on) return YH_EUNSUPPORTED;  // the SR step runs the standard branch
   yh_sr* sr = pl->sr;
   hipStream_t s = (hipStream_t)stream;
+  const long long count0 = sr->count;
   // history buffer for (c, phi) of each step
-  long long count0 = 0;
-  {
-    SrDev h;
-    YH_HIP(hipMemcpyAsync(&h, sr->st, sizeof h, hipMemcpyDeviceToHost, s));
-    YH_HIP(hipStreamSynchronize(s));
-    count0 = h.count;
-  }
   if (c_phi_out && count0 + nsteps > sr->histcap) {
     long long cap = std::max<long long>(count0 + nsteps, 2 * sr->histcap);
     double* nh = nullptr;
@@ -640,32 +793,29 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
     }
     sr->hist = nh;
     sr->histcap = cap;
-    if (sr->exec) {  // the captured step holds the old history pointer
-      (void)hipGraphExecDestroy(sr->exec);
-      sr->exec = nullptr;
-      (void)hipGraphDestroy(sr->graph);
-      sr->graph = nullptr;
-    }
+    sr_drop_graphs(sr);  // the captured steps hold the old history pointer
   }
   int rc = YH_OK;
   for (int q = 0; q < nsteps; ++q) {
-    const bool first = (count0 + q) == 0;
-    if (first || !sr->use_graph) {
-      rc = sr_enqueue(pl, first, s);
+    const long long cnt = count0 + q;
+    const int slot = (int)(cnt & 1);
+    if (cnt == 0 || !sr->use_graph) {
+      rc = sr_enqueue(pl, cnt == 0, slot, s);
       if (rc) return rc;
+      sr->count = cnt + 1;
       continue;
     }
-    if (!sr->exec || sr->graph_cur != pl->cur) {
-      if (sr->exec) {
-        (void)hipGraphExecDestroy(sr->exec);
-        (void)hipGraphDestroy(sr->graph);
-        sr->exec = nullptr;
-        sr->graph = nullptr;
+    if (!sr->exec[slot] || sr->graph_cur[slot] != pl->cur) {
+      if (sr->exec[slot]) {
+        (void)hipGraphExecDestroy(sr->exec[slot]);
+        (void)hipGraphDestroy(sr->graph[slot]);
+        sr->exec[slot] = nullptr;
+        sr->graph[slot] = nullptr;
       }
       if (!sr->cap_stream) YH_HIP(hipStreamCreateWithFlags(&sr->cap_stream, hipStreamNonBlocking));
       YH_HIP(hipStreamSynchronize(s));
       YH_HIP(hipStreamBeginCapture(sr->cap_stream, hipStreamCaptureModeThreadLocal));
-      rc = sr_enqueue(pl, false, sr->cap_stream);
+      rc = sr_enqueue(pl, false, slot, sr->cap_stream);
       hipGraph_t gph = nullptr;
       hipError_t e = hipStreamEndCapture(sr->cap_stream, &gph);
       if (rc) {
@@ -673,11 +823,12 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
         return rc;
       }
       YH_HIP(e);
-      sr->graph = gph;
-      YH_HIP(hipGraphInstantiate(&sr->exec, sr->graph, nullptr, nullptr, 0));
-      sr->graph_cur = pl->cur;
+      sr->graph[slot] = gph;
+      YH_HIP(hipGraphInstantiate(&sr->exec[slot], sr->graph[slot], nullptr, nullptr, 0));
+      sr->graph_cur[slot] = pl->cur;
     }
-    YH_HIP(hipGraphLaunch(sr->exec, s));
+    YH_HIP(hipGraphLaunch(sr->exec[slot], s));
+    sr->count = cnt + 1;
   }
   pl->count += nsteps;
   pl->have_prev = false;
@@ -694,7 +845,7 @@ int yh_sr_info_get(yh_plan* pl, yh_sr_info* out) {
   if (!pl->sr || !pl->sr->begun) return YH_ESTATE;
   SrDev h;
   YH_HIP(hipDeviceSynchronize());
-  YH_HIP(hipMemcpy(&h, pl->sr->st, sizeof h, hipMemcpyDeviceToHost));
+  YH_HIP(hipMemcpy(&h, pl->sr->st + (pl->sr->count & 1), sizeof h, hipMemcpyDeviceToHost));
   for (int q = 0; q < 3; ++q) {
     out->c[q] = h.c[q];
     out->phi[q] = h.phi[q];

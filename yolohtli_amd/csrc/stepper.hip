@@ -18,6 +18,7 @@
 //                  crosses waves through a small double-buffered LDS slot.
 //                  Halo cells outside the domain are loaded by reflection, so
 //                  the no-flux boundary costs nothing inside the substep loop.
+#include "direct.h"
 #include "tile.h"
 #include "yh_internal.h"
 
@@ -55,34 +56,6 @@ hipError_t launch_codes(const uint8_t* solid, uint8_t* codes, int nx, int ny, hi
 }
 
 // ------------------------------------------------------------------ direct
-// One step, one thread per cell.  Diffusion in the coefficient form
-// sum_dir c_dir*(f_dir - f) with c in {0,1,2} (reactionDiffusion.cu:162-173);
-// c_W = c_E = c_N = c_S = 1 without a mask (:188-197).
-template <int MODEL>
-__device__ __forceinline__ void cell_update(const KParams& k, const float* const* f, int c, int W,
-                                            int E, int N, int S, float cW, float cE, float cN,
-                                            float cS, bool sc, float* xo, float* inc) {
-  using M = Model<MODEL>;
-  float x[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-  for (int a = 0; a < M::NV; ++a) x[a] = f[a][c];
-  float lap[2] = {0.f, 0.f};
-#pragma unroll
-  for (int a = 0; a < M::ND; ++a) {
-    const float fc = x[a];
-    const float lx = cW * (f[a][W] - fc) + cE * (f[a][E] - fc);
-    const float ly = cN * (f[a][N] - fc) + cS * (f[a][S] - fc);
-    lap[a] = k.trx * lx + k.try_ * ly;
-  }
-  float r[3] = {0.f, 0.f, 0.f};
-  M::react(k, x[0], x[1], x[2], r[0], r[1], r[2]);
-  inc[0] = lap[0] + r[0];
-  inc[1] = (MODEL == M2V ? k.rs * lap[M::ND - 1] : 0.0f) + r[1];
-  inc[2] = r[2];
-#pragma unroll
-  for (int a = 0; a < M::NV; ++a) xo[a] = sc ? x[a] + inc[a] : 0.0f;
-}
-
 template <int MODEL, bool MASKED>
 __global__ __launch_bounds__(256) void k_direct(const KParams k, CFields in, Fields out,
                                                 Fields vt, const uint8_t* __restrict__ solid) {
@@ -121,23 +94,9 @@ __global__ __launch_bounds__(256) void k_direct_codes(const KParams k, CFields i
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= k.nx || j >= k.ny) return;
-  const int nx = k.nx, ny = k.ny;
-  const int c = i + nx * j;
-  const int W = mirror(i - 1, nx) + nx * j, E = mirror(i + 1, nx) + nx * j;
-  const int N = i + nx * mirror(j + 1, ny), S = i + nx * mirror(j - 1, ny);
-  float cW = 1.f, cE = 1.f, cN = 1.f, cS = 1.f;
-  bool sc = true;
-  if (codes) {
-    const uint32_t b = codes[c];
-    sc = b != kHole;
-    const uint32_t q = sc ? b : 0u;
-    cW = (float)(q & 3u);
-    cE = (float)((q >> 2) & 3u);
-    cN = (float)((q >> 4) & 3u);
-    cS = (float)((q >> 6) & 3u);
-  }
+  const int c = i + k.nx * j;
   float xo[3], inc[3];
-  cell_update<MODEL>(k, in.f, c, W, E, N, S, cW, cE, cN, cS, sc, xo, inc);
+  const bool sc = cell_update_codes<MODEL>(k, in.f, codes, i, j, xo, inc);
 #pragma unroll
   for (int a = 0; a < M::NV; ++a) out.f[a][c] = xo[a];
   if (APD && sc) {

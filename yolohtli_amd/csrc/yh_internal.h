@@ -209,30 +209,28 @@ __device__ __forceinline__ void tip_record(int i, int j, float tx, float ty, flo
   }
 }
 
-// Tips of cell (i, j) between the isolines u = Uth of past and present.
-// masked_disk (the solidSwitch branch, tipTracker.cu:47-51): only cells of
-// the disk (i-nx/2)^2 + (j-ny/2)^2 < r2, both roots recorded without a
-// discriminant test (:92-117); otherwise the interior 1 <= i < nx-2,
-// 1 <= j < ny-2, roots recorded when disc >= 0 (:126-197).
-__device__ __forceinline__ void tip_cell(int i, int j, int nx, int ny, bool masked_disk, int r2,
-                                         double Uth, const float* __restrict__ past,
-                                         const float* __restrict__ present, float t,
-                                         yh_tip* tips, int* keys, int* count, int cap) {
-#pragma clang fp contract(off)
-  bool inside;
+// Whether cell (i, j) takes part in the tip search.  masked_disk (the
+// solidSwitch branch, tipTracker.cu:47-51): only cells of the disk
+// (i-nx/2)^2 + (j-ny/2)^2 < r2; otherwise the interior 1 <= i < nx-2,
+// 1 <= j < ny-2 (:126).
+__device__ __forceinline__ bool tip_inside(int i, int j, int nx, int ny, bool masked_disk, int r2) {
   if (masked_disk) {
     const int ic = i - nx / 2, jc = j - ny / 2;
-    inside = (ic * ic + jc * jc) < r2;
-  } else {
-    inside = (i >= 1) && (i < nx - 2) && (j >= 1) && (j < ny - 2);
+    return (ic * ic + jc * jc) < r2;
   }
-  if (!inside) return;
-  const int s0 = i + nx * j;
-  const int sx = (i < nx - 1) ? s0 + 1 : s0;
-  const int sy = (j < ny - 1) ? s0 + nx : s0;
-  const int sxy = ((j < ny - 1) && (i < nx - 1)) ? s0 + nx + 1 : s0;
-  const double x1 = present[s0], x2 = present[sx], x4 = present[sy], x3 = present[sxy];
-  const double y1 = past[s0], y2 = past[sx], y4 = past[sy], y3 = past[sxy];
+  return (i >= 1) && (i < nx - 2) && (j >= 1) && (j < ny - 2);
+}
+
+// Tips of cell (i, j) between the isolines u = Uth of past and present from
+// the corner values (1 = (i,j), 2 = east, 3 = north-east, 4 = north, each
+// replaced by the cell itself past the last column/row).  masked_disk: both
+// roots recorded without a discriminant test (tipTracker.cu:92-117);
+// otherwise roots recorded when disc >= 0 (:126-197).
+__device__ __forceinline__ void tip_solve(int i, int j, int nx, bool masked_disk, double Uth,
+                                          double x1, double x2, double x3, double x4, double y1,
+                                          double y2, double y3, double y4, float t, yh_tip* tips,
+                                          int* keys, int* count, int cap) {
+#pragma clang fp contract(off)
   const double x3y1 = x3 * y1, x4y1 = x4 * y1, x3y2 = x3 * y2, x4y2 = x4 * y2;
   const double x1y3 = x1 * y3, x2y3 = x2 * y3, x1y4 = x1 * y4, x2y4 = x2 * y4;
   const double x2y1 = x2 * y1, x1y2 = x1 * y2, x4y3 = x4 * y3, x3y4 = x3 * y4;
@@ -246,10 +244,25 @@ __device__ __forceinline__ void tip_cell(int i, int j, int nx, int ny, bool mask
   const double px = ctn2 - Uth * ctn1;
   const double py = Uth * ctn1 - x3y1 + x4y2 + x1y3 - x2y4 + 2.0 * (x2y1 - x1y2);
   if (!masked_disk && !(disc >= 0.0)) return;
+  const int s0 = i + nx * j;
   tip_record(i, j, (float)((px + disc) / den1), (float)((py + disc) / den2), t, 2 * s0, tips,
              keys, count, cap);
   tip_record(i, j, (float)((px - disc) / den1), (float)((py - disc) / den2), t, 2 * s0 + 1, tips,
              keys, count, cap);
+}
+
+// Tips of cell (i, j) between the isolines u = Uth of past and present.
+__device__ __forceinline__ void tip_cell(int i, int j, int nx, int ny, bool masked_disk, int r2,
+                                         double Uth, const float* __restrict__ past,
+                                         const float* __restrict__ present, float t,
+                                         yh_tip* tips, int* keys, int* count, int cap) {
+  if (!tip_inside(i, j, nx, ny, masked_disk, r2)) return;
+  const int s0 = i + nx * j;
+  const int sx = (i < nx - 1) ? s0 + 1 : s0;
+  const int sy = (j < ny - 1) ? s0 + nx : s0;
+  const int sxy = ((j < ny - 1) && (i < nx - 1)) ? s0 + nx + 1 : s0;
+  tip_solve(i, j, nx, masked_disk, Uth, present[s0], present[sx], present[sxy], present[sy],
+            past[s0], past[sx], past[sxy], past[sy], t, tips, keys, count, cap);
 }
 
 // Stencil code of one cell, built once per mask: 2-bit coefficients
