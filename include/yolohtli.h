@@ -92,7 +92,9 @@ typedef struct yh_derived {
 void yh_params_default(yh_params* p);
 /* Fenton-Karma 1998 Beeler-Reuter parameter set. */
 void yh_fk_params_default(yh_fk_params* fk);
-/* Validate and derive (YH_EINVAL for a timeIntOrder other than 1, 2, 4). */
+/* Validate and derive (YH_EINVAL for a timeIntOrder other than 1, 2, 4, for
+ * nx or ny below 4, or for more than 2^30 cells: cell indices are int32 and
+ * the tip and contour keys 2*cell + root must fit in one). */
 int yh_params_derive(const yh_params* p, yh_derived* d);
 
 /* ------------------------------------------------------------- workloads */

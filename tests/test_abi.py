@@ -70,6 +70,7 @@ def test_derive_matches_oracle(over):
     (dict(timeIntOrder=0), T.YH_EINVAL),
     (dict(dt=2.0), T.YH_EUNSTABLE),
     (dict(nx=2), T.YH_EINVAL),
+    (dict(nx=32769, ny=32768), T.YH_EINVAL),
     (dict(model=7), T.YH_EINVAL),
     (dict(model=1, fk=dict(tau_d=0.0)), T.YH_EINVAL),
 ])

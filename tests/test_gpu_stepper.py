@@ -242,3 +242,44 @@ def test_4096_tiled_equals_one_step_kernel(torch_cuda):
         d = np.abs(out[0][a] - out[1][a])
         print(a, d.max(), float(np.mean(d > 1e-5)))
         assert np.mean(d > 1e-5) < 1e-4 and np.median(d) < 1e-7
+
+
+def test_max_grid_tiled_equals_one_step_kernel(torch_cuda):
+    """The largest grid the ABI accepts, 2^30 cells (32768^2, 4 GiB per
+    variable, so byte offsets pass 2^32), at the 1024^2 config's hole density
+    (204800 holes): the headline tile shape against the one-step kernel over
+    16 steps; holes stay zero, and the last rows (the largest offsets) agree
+    as the first rows do."""
+    n = 32768
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    with pytest.raises(RuntimeError):
+        Y.Plan(T.grid_params(n + 1, n, model=T.YH_MODEL_3V_FK, solidSwitch=1))
+    mask = Y.make_holes(n, n, 200 * 1024, 3.0, 12.0, Y.seed(1, 1, 3))
+    f = Y.init_random(n, n, Y.seed(1, 1, 4), gate=1.0, nvars=3)
+    out = []
+    for cfg in ((T.YH_KERNEL_TILED, 8, 8, 8, 1), (T.YH_KERNEL_DIRECT, 1, 0, 0, 1)):
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        pl.set_kernel(*cfg)
+        pl.step(16)
+        assert pl.check_finite() == 0
+        out.append(pl.get_state())
+        pl.close()
+    del f
+    holes = np.flatnonzero(mask == 0)
+    del mask
+    edge = 64 * n
+    for a in range(3):
+        assert not np.any(out[0][a][holes]) and not np.any(out[1][a][holes])
+        d = np.abs(out[0][a] - out[1][a])
+        # an offset bug would spoil whole rows; a cell whose u sits within
+        # rounding of a gate threshold may switch in one kernel only
+        far = float(np.count_nonzero(d[-edge:] > 1e-5)) / edge
+        near = float(np.count_nonzero(d[:edge] > 1e-5)) / edge
+        frac = float(np.count_nonzero(d > 1e-5)) / d.size
+        med = float(np.median(d[::1021]))
+        print(a, float(d.max()), near, far, frac, med, flush=True)
+        assert frac < 1e-4 and med < 1e-7
+        assert far < 1e-3 and near < 1e-3
+        del d

@@ -169,7 +169,8 @@ void yh_params_default(yh_params* p) {
 int yh_params_derive(const yh_params* p, yh_derived* d) {
   if (!p || !d) return YH_EINVAL;
   if (p->nx < 4 || p->ny < 4) return YH_EINVAL;
-  if ((int64_t)p->nx * p->ny > (int64_t)1 << 31) return YH_EINVAL;
+  // cell indices are int32 and tip/contour keys 2*cell + root must fit too
+  if ((int64_t)p->nx * p->ny > (int64_t)1 << 30) return YH_EINVAL;
   if (!(p->Lx > 0) || !(p->Ly > 0) || !(p->dt > 0)) return YH_EINVAL;
   if (p->model != YH_MODEL_2V_REF && p->model != YH_MODEL_3V_FK) return YH_EINVAL;
   if (p->timeIntOrder != 1 && p->timeIntOrder != 2 && p->timeIntOrder != 4) return YH_EINVAL;

@@ -106,6 +106,7 @@ int yh_contour_dev(int nx, int ny, int mode, double th1, double th2, double th3,
       cap < 0)
     return YH_EINVAL;
   if (mode != 1 && !field1) return YH_EINVAL;
+  if ((int64_t)nx * ny > (int64_t)1 << 30) return YH_EINVAL;  // keys 2*cell + sub in int32
   hipStream_t s = (hipStream_t)stream;
   // countour_wrapper zeroes the count before every search (spaceAPD.cu:261)
   YH_HIP(hipMemsetAsync(d_count, 0, sizeof(int), s));
