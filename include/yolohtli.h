@@ -207,9 +207,11 @@ int yh_check_finite(yh_plan* plan, void* stream);
  * While on, the plan steps on the per-stage path. */
 int yh_set_stimulus(yh_plan* plan, int on, int x, int y);
 
-/* Plan options.  YH_OPT_KEEP_PREV (default 1): end every yh_step with a single
- * step so the previous state is kept for yh_track_tips; 0 lets yh_step use
- * only k-substep launches (no tip tracking until the next yh_step with 1). */
+/* Plan options.  YH_OPT_KEEP_PREV (default 1): every yh_step keeps u of the
+ * step before its last for yh_track_tips (the last k-substep launch writes
+ * it, or, for tile shapes built without that output and with APD on, a
+ * single-step tail leaves it in the other state buffer); 0 drops it (no tip
+ * tracking until the next yh_step with 1). */
 enum { YH_OPT_KEEP_PREV = 1 };
 int yh_set_option(yh_plan* plan, int option, int64_t value);
 

@@ -169,6 +169,55 @@ def test_tips_match_oracle_and_plan_log(torch_cuda):
     np.testing.assert_allclose(got[order], want, rtol=0, atol=1e-5)
 
 
+@pytest.mark.parametrize("masked", [False, True])
+@pytest.mark.parametrize("cfg", [(T.YH_KERNEL_TILED, 8, 8, 8, 1), (T.YH_KERNEL_TILED, 8, 8, 4, 1)])
+def test_tips_after_multistep_call_match_oracle(torch_cuda, cfg, masked):
+    """Tips after a call of 37 steps with K = 8 per launch.  The previous u
+    comes from the last launch (8 rows x 8 waves: it writes u one substep
+    before its end) or from a single-step tail (8 rows x 4 waves).  A second
+    plan takes 36 steps without keeping the previous state, then 1, and
+    exposes steps 3036 and 3037: the first plan's final state equals the
+    second's bit for bit, and its tips are the oracle's tips between those
+    two states.  With a mask the search covers the centred disk only
+    (tipTracker.cu:47-51)."""
+    nx = ny = 256
+    # with a mask a disk of radius 100 (bounding box [29, 228)^2)
+    p = T.grid_params(nx, ny, solidSwitch=int(masked), tipOffsetX=100, tipOffsetY=100)
+    mask = Y.make_holes(nx, ny, 6, 3.0, 6.0, Y.seed(9, 0, 1)) if masked else None
+    plans = []
+    for _ in range(2):
+        pl = Y.Plan(p)
+        if masked:
+            pl.set_mask(mask)
+        pl.set_state(*Y.initGates(nx, ny))
+        pl.set_kernel(*cfg)
+        pl.step(3000)
+        plans.append(pl)
+    a, b = plans
+    a.step(37)
+    a.track_tips()
+    log = a.read_tips()
+    fa = a.get_state()
+    # 36 steps in K-substep launches (a tiled cell update does not depend on
+    # where launches split), then the last step as the first plan takes it:
+    # inside a tiled launch (fused) or as the single-step tail
+    b.set_option(T.YH_OPT_KEEP_PREV, 0)
+    b.step(36)
+    s36 = b.get_state()
+    b.set_option(T.YH_OPT_KEEP_PREV, 1)
+    b.step(1)
+    s37 = b.get_state()
+    for pl in plans:
+        pl.close()
+    want, _ = O.tips(p, s36[0].astype(np.float64), s37[0].astype(np.float64),
+                     O.derive(p).dt * 3037)
+    assert len(want) > 0
+    for x, y in zip(fa, s37):
+        np.testing.assert_array_equal(x, y)
+    assert len(log) == len(want)
+    np.testing.assert_allclose(log, want, rtol=0, atol=1e-5)
+
+
 def test_check_finite_and_argument_errors(torch_cuda):
     p = T.grid_params(64, 64)
     pl = Y.Plan(p)

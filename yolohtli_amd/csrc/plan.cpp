@@ -64,19 +64,27 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
       done += 1;
     }
     pl->have_prev = true;
+    pl->past_prev_u = false;
     pl->count += nsteps;
     return YH_OK;
   }
-  const int tail = pl->keep_prev ? 1 : 0;
-  if (c.variant == YH_KERNEL_TILED) {
+  // keep_prev: the last tiled launch writes u one substep before its end
+  // (shapes built with that output, no APD), else a single-step tail
+  const bool tiled = c.variant == YH_KERNEL_TILED;
+  const bool prev_fused =
+      tiled && pl->keep_prev && !ap && tiled_prev_supported(c.rows, c.waves, c.cols);
+  const int tail = (pl->keep_prev && !prev_fused) ? 1 : 0;
+  if (prev_fused && !pl->prev_u) YH_HIP(hipMalloc(&pl->prev_u, pl->n * sizeof(float)));
+  if (tiled) {
     while (remaining > tail) {
       const int K = std::min(c.substeps, remaining - tail);
       CFields ci;
       Fields co;
       fields(pl->cur, &ci, &co);
       apd.count0 = (int)done;
+      float* prev = (prev_fused && K == remaining) ? pl->prev_u : nullptr;
       YH_HIP(launch_tiled(pl->p.model, masked, c.rows, c.waves, c.cols, K, pl->k, ci, co,
-                          masked ? pl->codes_d : nullptr, s, ap));
+                          masked ? pl->codes_d : nullptr, s, ap, prev));
       pl->cur ^= 1;
       remaining -= K;
       done += K;
@@ -93,7 +101,8 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
     remaining -= 1;
     done += 1;
   }
-  pl->have_prev = c.variant != YH_KERNEL_TILED || pl->keep_prev || c.substeps == 1;
+  pl->have_prev = !tiled || pl->keep_prev || c.substeps == 1;
+  pl->past_prev_u = prev_fused;
   pl->count += nsteps;
   return YH_OK;
 }
@@ -109,6 +118,7 @@ static void plan_free(yh_plan* pl) {
   if (pl->keys_d) (void)hipFree(pl->keys_d);
   if (pl->tipcount_d) (void)hipFree(pl->tipcount_d);
   if (pl->flag_d) (void)hipFree(pl->flag_d);
+  if (pl->prev_u) (void)hipFree(pl->prev_u);
   void* apd[] = {pl->apd.front, pl->apd.back, pl->apd.apd1, pl->apd.apd2, pl->apd.first};
   for (void* q : apd)
     if (q) (void)hipFree(q);
@@ -249,8 +259,9 @@ int yh_track_tips(yh_plan* pl, void* stream) {
   if (!pl->have_prev) return YH_ESTATE;
   // main.cu:961-964 after swapSoA: g_past = previous state, g_present = newest.
   const double t = pl->d.dt * (double)pl->count;
+  const float* past = pl->past_prev_u ? pl->prev_u : pl->buf[pl->cur ^ 1][0];
   YH_HIP(launch_tips(pl->k, pl->p.solidSwitch, pl->p.tipOffsetX, pl->p.tipOffsetY, pl->p.Uth,
-                     pl->buf[pl->cur ^ 1][0], pl->buf[pl->cur][0], t, pl->tips_d, pl->keys_d,
+                     past, pl->buf[pl->cur][0], t, pl->tips_d, pl->keys_d,
                      pl->tipcount_d, pl->tipcap, (hipStream_t)stream));
   return YH_OK;
 }

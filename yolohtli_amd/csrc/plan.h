@@ -16,7 +16,9 @@ struct yh_plan {
   int device = 0;
   float* buf[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   int cur = 0;             // buf[cur] holds the current state
-  bool have_prev = false;  // buf[cur^1] holds the state one step earlier
+  bool have_prev = false;  // u one step earlier is known: in prev_u or buf[cur^1][0]
+  bool past_prev_u = false;  // it is in prev_u (written by the last tiled launch)
+  float* prev_u = nullptr;   // lazily allocated, n floats
   uint8_t* solid_d = nullptr;
   uint8_t* codes_d = nullptr;
   bool mask_set = false;
@@ -47,8 +49,9 @@ namespace yh {
 inline bool plan_general(const yh_plan* pl) { return !standard_branch(pl->p) || pl->stim_on; }
 // Resolve YH_KERNEL_AUTO to a concrete configuration for this plan.
 yh_kernel_cfg resolve_cfg(const yh_plan* pl);
-// Advance `nsteps` on the plan's buffers; the last step is a single step so
-// that buf[cur^1] is the previous state.
+// Advance `nsteps` on the plan's buffers, keeping u of the step before the
+// last (for the tip test) when keep_prev: the last tiled launch writes it to
+// prev_u, or a single-step tail leaves it in buf[cur^1].
 int plan_advance(yh_plan* pl, int nsteps, hipStream_t s);
 void sr_free(yh_sr* sr);
 }  // namespace yh

@@ -18,6 +18,8 @@
 //                  crosses waves through a small double-buffered LDS slot.
 //                  Halo cells outside the domain are loaded by reflection, so
 //                  the no-flux boundary costs nothing inside the substep loop.
+#include <algorithm>
+
 #include "direct.h"
 #include "tile.h"
 #include "yh_internal.h"
@@ -139,15 +141,16 @@ hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields o
 }
 
 // ------------------------------------------------------------------- tiled
-template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD>
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV>
 __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, Fields out,
                                                    const uint8_t* __restrict__ codes, int K,
-                                                   int tiles_x, int ntiles, const ApdState apd) {
+                                                   int tiles_x, int ntiles, const ApdState apd,
+                                                   float* __restrict__ prev) {
   __shared__ TileLds<MODEL, NW, CW> lds;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int ty = tile / tiles_x;
-  tile_advance<MODEL, MASKED, R, NW, CW, APD>(k, in, out, codes, K, tile - ty * tiles_x, ty, lds,
-                                              apd);
+  tile_advance<MODEL, MASKED, R, NW, CW, APD, PREV>(k, in, out, codes, K, tile - ty * tiles_x, ty,
+                                                    lds, apd, prev);
 }
 
 bool tiled_supported(int rows, int waves, int cols) {
@@ -163,25 +166,32 @@ bool tiled_apd_supported(int rows, int waves, int cols) {
   return cols == 1 && (rows == 4 || rows == 8) && (waves == 8 || waves == 16);
 }
 
-template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD>
+// Shapes built with the previous-u output (the same set).
+bool tiled_prev_supported(int rows, int waves, int cols) {
+  return tiled_apd_supported(rows, waves, cols);
+}
+
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV = false>
 static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out,
-                                 const uint8_t* codes, hipStream_t s, const ApdState& apd) {
+                                 const uint8_t* codes, hipStream_t s, const ApdState& apd,
+                                 float* prev = nullptr) {
   const int wout = 64 * CW - 2 * K, hout = NW * R - 2 * K;
   if (wout <= 0 || hout <= 0) return hipErrorInvalidValue;
   const int tiles_x = (k.nx + wout - 1) / wout;
   const int tiles_y = (k.ny + hout - 1) / hout;
   const int ntiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW, APD>), dim3(ntiles), dim3(64 * NW), 0, s,
-                     k, in, out, codes, K, tiles_x, ntiles, apd);
+  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV>), dim3(ntiles), dim3(64 * NW),
+                     0, s, k, in, out, codes, K, tiles_x, ntiles, apd, prev);
   return hipGetLastError();
 }
 
 template <int MODEL, bool MASKED>
 static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KParams& k,
                                  CFields in, Fields out, const uint8_t* codes, hipStream_t s,
-                                 const ApdState* apd) {
+                                 const ApdState* apd, float* prev) {
   const ApdState a = apd ? *apd : ApdState();
   if (apd) {
+    if (prev) return hipErrorInvalidValue;
 #define YH_TILE_APD(RR, WW)                    \
   if (rows == RR && waves == WW && cols == 1) \
     return launch_tiled_t<MODEL, MASKED, RR, WW, 1, true>(K, k, in, out, codes, s, a);
@@ -190,6 +200,17 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
     YH_TILE_APD(4, 16)
     YH_TILE_APD(8, 16)
 #undef YH_TILE_APD
+    return hipErrorInvalidValue;
+  }
+  if (prev) {
+#define YH_TILE_PREV(RR, WW)                   \
+  if (rows == RR && waves == WW && cols == 1) \
+    return launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, true>(K, k, in, out, codes, s, a, prev);
+    YH_TILE_PREV(4, 8)
+    YH_TILE_PREV(8, 8)
+    YH_TILE_PREV(4, 16)
+    YH_TILE_PREV(8, 16)
+#undef YH_TILE_PREV
     return hipErrorInvalidValue;
   }
 #define YH_TILE_CASE(RR, WW, CC)                 \
@@ -214,22 +235,24 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
 
 hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
                         const KParams& k, CFields in, Fields out, const uint8_t* codes,
-                        hipStream_t s, const ApdState* apd) {
+                        hipStream_t s, const ApdState* apd, float* prev) {
   if (model == M2V) {
-    return masked ? launch_tiled_m<M2V, true>(rows, waves, cols, K, k, in, out, codes, s, apd)
-                  : launch_tiled_m<M2V, false>(rows, waves, cols, K, k, in, out, codes, s, apd);
+    return masked ? launch_tiled_m<M2V, true>(rows, waves, cols, K, k, in, out, codes, s, apd, prev)
+                  : launch_tiled_m<M2V, false>(rows, waves, cols, K, k, in, out, codes, s, apd, prev);
   }
-  return masked ? launch_tiled_m<MFK, true>(rows, waves, cols, K, k, in, out, codes, s, apd)
-                : launch_tiled_m<MFK, false>(rows, waves, cols, K, k, in, out, codes, s, apd);
+  return masked ? launch_tiled_m<MFK, true>(rows, waves, cols, K, k, in, out, codes, s, apd, prev)
+                : launch_tiled_m<MFK, false>(rows, waves, cols, K, k, in, out, codes, s, apd, prev);
 }
 
 // -------------------------------------------------------------------- tips
-__global__ void k_tips(int nx, int ny, int masked_disk, int r2, double Uth,
+// Launched over the cells the tip test can accept: from (i0, j0), the
+// centred disk's bounding box with a mask, the whole grid without.
+__global__ void k_tips(int nx, int ny, int i0, int j0, int masked_disk, int r2, double Uth,
                        const float* __restrict__ past, const float* __restrict__ present,
                        float t_host, const long long* d_count, double dt, yh_tip* tips,
                        int* keys, int* count, int cap) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = j0 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= nx || j >= ny) return;
   // physicalTime = dt*count (main.cu:885), from the device step counter when given
   const float t = d_count ? (float)(dt * (double)(*d_count)) : t_host;
@@ -239,10 +262,21 @@ __global__ void k_tips(int nx, int ny, int masked_disk, int r2, double Uth,
 hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int tipOffsetY,
                        double Uth, const float* past, const float* present, double time,
                        yh_tip* tips, int* keys, int* count, int cap, hipStream_t s) {
-  dim3 b(64, 4), g((k.nx + 63) / 64, (k.ny + 3) / 4);
-  hipLaunchKernelGGL(k_tips, g, b, 0, s, k.nx, k.ny, masked_disk, tipOffsetX * tipOffsetY, Uth,
-                     past, present, (float)time, (const long long*)nullptr, 0.0, tips, keys,
-                     count, cap);
+  const int r2 = tipOffsetX * tipOffsetY;
+  int i0 = 0, j0 = 0, i1 = k.nx, j1 = k.ny;  // [i0, i1) x [j0, j1)
+  if (masked_disk) {
+    if (r2 <= 0) return hipSuccess;  // the disk is empty
+    // (i - nx/2)^2 + (j - ny/2)^2 < r2 needs |i - nx/2| <= R with R^2 < r2
+    int R = 0;
+    while ((long long)(R + 1) * (R + 1) < r2) ++R;
+    i0 = std::max(0, k.nx / 2 - R);
+    j0 = std::max(0, k.ny / 2 - R);
+    i1 = std::min(k.nx, k.nx / 2 + R + 1);
+    j1 = std::min(k.ny, k.ny / 2 + R + 1);
+  }
+  dim3 b(64, 4), g((i1 - i0 + 63) / 64, (j1 - j0 + 3) / 4);
+  hipLaunchKernelGGL(k_tips, g, b, 0, s, k.nx, k.ny, i0, j0, masked_disk, r2, Uth, past, present,
+                     (float)time, (const long long*)nullptr, 0.0, tips, keys, count, cap);
   return hipGetLastError();
 }
 

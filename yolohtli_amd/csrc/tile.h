@@ -35,12 +35,15 @@ struct TileLds {
 
 // With APD, every substep also updates the action-potential-duration state of
 // the cells this tile outputs (their value is exact at every substep), so
-// the measurement costs no extra launch or state traffic.
-template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false>
+// the measurement costs no extra launch or state traffic.  With PREV, u of
+// the output cells one substep before the end is written to prev_u (0 in
+// holes): the previous state the tip test needs, without a single-step tail.
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false, bool PREV = false>
 __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Fields out,
                                              const uint8_t* __restrict__ codes, int K, int tx,
                                              int ty, TileLds<MODEL, NW, CW>& lds,
-                                             const ApdState& apd = ApdState()) {
+                                             const ApdState& apd = ApdState(),
+                                             float* __restrict__ prev_u = nullptr) {
   using M = Model<MODEL>;
   constexpr int NV = M::NV, ND = M::ND;
   constexpr int WR = 64 * CW;
@@ -221,6 +224,17 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
             apd_cell(u_old, X[0][r][c], apd.count0 + s + 1, apd.dt, apd.th,
                      a_front[APD ? r : 0][c], a_back[APD ? r : 0][c], a_d1[APD ? r : 0][c],
                      a_d2[APD ? r : 0][c], a_first[APD ? r : 0][c]);
+          }
+          if (PREV && s == K - 1) {
+            // u of an output cell one substep before the end (the rows
+            // active in the last substep are exactly the output rows)
+            int px = gx0 + c, py = gy0 + r;
+            asm volatile("" : "+v"(px), "+s"(py));  // addresses computed here, not hoisted
+            const int lc = lane * CW + c;
+            if (lc >= K && lc < WR - K && px < nx && py < ny) {
+              const bool h = MASKED && ((hole[c] >> r) & 1u);
+              prev_u[(size_t)py * nx + px] = h ? 0.0f : u_old;
+            }
           }
         }
       }

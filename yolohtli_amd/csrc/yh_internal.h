@@ -301,11 +301,13 @@ hipError_t launch_direct(int model, const KParams& k, CFields in, Fields out, Fi
 hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields out,
                                const uint8_t* codes, hipStream_t s,
                                const ApdState* apd = nullptr);
+// prev (optional): u of the output cells after K-1 substeps, for the tip test.
 hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
                         const KParams& k, CFields in, Fields out, const uint8_t* codes,
-                        hipStream_t s, const ApdState* apd = nullptr);
+                        hipStream_t s, const ApdState* apd = nullptr, float* prev = nullptr);
 bool tiled_supported(int rows, int waves, int cols);
 bool tiled_apd_supported(int rows, int waves, int cols);
+bool tiled_prev_supported(int rows, int waves, int cols);
 hipError_t launch_tips(const KParams& k, int masked_disk, int tipOffsetX, int tipOffsetY,
                        double Uth, const float* past, const float* present, double time,
                        yh_tip* tips, int* keys, int* count, int cap, hipStream_t s);
