@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--adj-fd-taus", type=lambda s: [int(x) for x in s.split(",")],
                     default=[0, 4])
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant-kernel timing (for profiling); needs --cfg")
     return ap.parse_args()
@@ -116,6 +118,65 @@ def copy_bandwidth(stream, mib=256, reps=20):
     ms = e0.elapsed_time(e1) / reps
     del src, dst
     return 2 * (mib << 20) / (ms / 1e3) / 1e9
+
+
+def _counter_pass(a, cfg, counter, launches=50):
+    """One rocprofv3 pass with a single counter over the --kernel-only run of
+    this configuration (the program itself follows "--"; KILL after 120 s).
+    Returns {kernel name: [values per dispatch]} or None."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    d = tempfile.mkdtemp(prefix="yh_traffic_", dir="/tmp")
+    cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--kernel-trace",
+           "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+           os.path.abspath(__file__), "--kernel-only", "--cfg", ",".join(str(x) for x in cfg[1:]),
+           "--kernel-launches", str(launches), "--nx", str(a.nx), "--holes", str(a.holes),
+           "--model", a.model]
+    try:
+        subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=150)
+    except (subprocess.TimeoutExpired, OSError):
+        pass
+    out = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") == counter:
+                    out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    shutil.rmtree(d, ignore_errors=True)
+    return out or None
+
+
+def hbm_traffic(a, cfg):
+    """HBM bytes per launch of the dominant kernel: FETCH_SIZE and WRITE_SIZE
+    (KiB per dispatch, one rocprofv3 pass each), each scaled by what the same
+    counter reads for the 256 MiB device copy the --kernel-only run makes
+    first (256 MiB read and 256 MiB written).  None if a pass fails."""
+    total = 0.0
+    parts = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        rows = _counter_pass(a, cfg, counter)
+        if not rows:
+            return None
+        kern = [v for k, vs in rows.items() if "k_tiled" in k for v in vs]
+        copy = sorted(v for k, vs in rows.items() if "copyBuffer" in k for v in vs)
+        if not kern or not copy:
+            return None
+        copy_kib = float(np.median(copy[-3:]))  # the three 256 MiB copies
+        if copy_kib <= 0:
+            return None
+        factor = (256 << 10) / copy_kib
+        kib = float(np.mean(kern[len(kern) // 5:]))  # after the first launches
+        parts[counter] = {"kib_per_dispatch": round(kib, 1), "copy_kib": round(copy_kib, 1),
+                          "scale": round(factor, 4)}
+        total += kib * factor * 1024.0
+    return total, parts
 
 
 def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell, launches=None):
@@ -423,6 +484,15 @@ def main():
         f"({n * K / (t_launch_ms / 1e3) / 1e9:.1f} Gcell-steps/s inside one launch)")
 
     pl.close()
+    traffic = None
+    if not a.no_traffic and cfg[0] == T.YH_KERNEL_TILED:
+        log("HBM traffic: FETCH_SIZE and WRITE_SIZE passes under rocprofv3 ...")
+        traffic = hbm_traffic(a, cfg)
+        if traffic:
+            log(f"HBM traffic {traffic[0] / 1e6:.2f} MB per launch "
+                f"({traffic[0] / alg_bytes:.3f}x the algorithmic {alg_bytes / 1e6:.2f} MB)")
+        else:
+            log("HBM traffic: counter passes unavailable")
     copy_gbps = copy_bandwidth(stream)
     log(f"device copy of 256 MiB: {copy_gbps:.0f} GB/s ({copy_gbps / HBM_PEAK_GBPS:.3f} of peak)")
     control = None
@@ -475,7 +545,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": round(traffic[0]) if traffic else None,
+            "traffic_counters": traffic[1] if traffic else None,
             "kernel": kname,
             "launch_us": round(t_launch_ms * 1e3, 3),
             "alg_bytes_per_launch": alg_bytes,
