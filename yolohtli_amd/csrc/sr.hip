@@ -77,7 +77,7 @@ struct SrArgs {
 };
 
 constexpr int kSrTipCap = 4096;
-constexpr int kSliceBX = 64, kSliceBY = 4;
+constexpr int kSliceBX = 64, kSliceBY = 8;
 
 __device__ __forceinline__ bool in_disk(const SrGeom& g, int i, int j, int cx, int cy) {
   const int a = i - g.nx / 2 - cx, b = j - g.ny / 2 - cy;
@@ -409,7 +409,7 @@ __device__ __forceinline__ BfIdx bf_index(uint32_t f, int c, int stride) {
 // advFDBFECC.cu:30-31) and neighbour flags are computed once.  Cells outside
 // the domain are never read: the boundary rules substitute an inside
 // neighbour.
-constexpr int kBfTX = 64, kBfTY = 8, kBfHalo = 3, kBfThreads = 512;
+constexpr int kBfTX = 64, kBfTY = 16, kBfHalo = 3, kBfThreads = 1024;
 constexpr int kBfLW = kBfTX + 2 * kBfHalo, kBfLH = kBfTY + 2 * kBfHalo;
 constexpr int kBfLN = kBfLW * kBfLH;
 
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CF
   const int nx = g.nx, ny = g.ny;
   const int i0 = blockIdx.x * kBfTX - kBfHalo, j0 = blockIdx.y * kBfTY - kBfHalo;
   const bool commit = blockIdx.x == 0 && blockIdx.y == 0;
-  // 1 the integrals (wave w sums columns w, w+8); the state tile and the
+  // 1 the integrals (wave w sums column w, w < 12 of the 16 waves); the state tile and the
   //   neighbour flags, which do not depend on the solve; in workgroup 0 the
   //   centre and the tip log slots
   for (int q = wv; q < 12; q += kBfThreads / 64) {
