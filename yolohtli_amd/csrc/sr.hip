@@ -2,10 +2,10 @@
 //
 // One SR step follows display() (main.cu:894-953) in three launches:
 //   1 k_sr_rdtips   RD X -> Xo (reactionDiffusion_kernel) and the tips
-//                   between Xo.u and X.u (tip_wrapper, main.cu:900); a block
-//                   of the tip search also updates its one-cell north/east
-//                   halo through the same code path into LDS, so every
-//                   corner value is the one its own cell stores
+//                   between Xo.u and X.u (tip_wrapper, main.cu:900); blocks
+//                   overlap by one column and row, so every corner of an
+//                   owned cell is updated in the block, by the same code
+//                   as in the block that owns it
 //   2 k_sr_slices   every workgroup picks the disk centre (last tip of this
 //                   step in (cell, root) order, tipx0 at count 0,
 //                   symmetryReduction.cu:95-107), then over the disk's
@@ -85,51 +85,46 @@ __device__ __forceinline__ bool in_disk(const SrGeom& g, int i, int j, int cx, i
 }
 
 // ------------------------------------------------------------- RD + tips
-// One thread per cell of a 64 x 4 block: the RD step of the cell; a block
-// holding cells of the tip search also updates its east column and north row
-// (69 halo cells, the same loop body, so each value equals what the cell's
-// own thread stores) into LDS, then runs the tip test between the new and
-// the old u (g_past = RD output, g_present = old state: main.cu:900,
-// tipTracker.cu:586).
-constexpr int kRtW = 65, kRtH = 5;  // the block's 64 x 4 cells + east column + north row
+// Blocks of 64 x 4 threads overlap by one column and one row: a block owns
+// its first 63 x 3 cells and computes the RD step of all 64 x 4 (every thread
+// exactly one), so the east, north and north-east corners of each owned cell
+// are in the block's LDS.  A cell in the overlap is computed by two blocks
+// with the same code, bit for bit, and stored by its owner.  Then the tip
+// test between the new and the old u (g_past = RD output, g_present = old
+// state: main.cu:900, tipTracker.cu:586).
+constexpr int kRtBX = 64, kRtBY = 4;          // threads per block
+constexpr int kRtOX = kRtBX - 1, kRtOY = kRtBY - 1;  // cells owned per block
 
 template <int MODEL, bool MASKED>
-__global__ __launch_bounds__(256) void k_sr_rdtips(const KParams k, const SrGeom g, CFields X,
-                                                   Fields Xo, const uint8_t* __restrict__ codes,
-                                                   double Uth, const SrDev* __restrict__ st,
-                                                   yh_tip* tips, int* keys, int* count) {
+__global__ __launch_bounds__(kRtBX* kRtBY) void k_sr_rdtips(const KParams k, const SrGeom g,
+                                                           CFields X, Fields Xo,
+                                                           const uint8_t* __restrict__ codes,
+                                                           double Uth,
+                                                           const SrDev* __restrict__ st,
+                                                           yh_tip* tips, int* keys, int* count) {
   using M = Model<MODEL>;
-  __shared__ float s_u[kRtH * kRtW];
+  __shared__ float s_u[kRtBY * kRtBX];
   const int nx = g.nx, ny = g.ny;
-  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 64 * ty;
-  const int i = i0 + tx, j = j0 + ty;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int i = blockIdx.x * kRtOX + tx, j = blockIdx.y * kRtOY + ty;
   const bool valid = i < nx && j < ny;
-  const bool tip = valid && tip_inside(i, j, nx, ny, MASKED, g.r2);
-  const bool halo = __syncthreads_or(tip) && tid < kRtW + kRtH - 1;
-  // q = 0: the cell; q = 1: halo cell tid (north row, then east column)
-  const int nq = halo ? 2 : 1;
-#pragma nounroll
-  for (int q = 0; q < nq; ++q) {
-    const int li = q == 0 ? tx : (tid < kRtW ? tid : kRtW - 1);
-    const int lj = q == 0 ? ty : (tid < kRtW ? kRtH - 1 : tid - kRtW);
-    const int ii = i0 + li, jj = j0 + lj;
-    if (ii < nx && jj < ny) {
-      float xo[3], inc[3];
-      cell_update_codes<MODEL>(k, X.f, MASKED ? codes : nullptr, ii, jj, xo, inc);
-      if (q == 0) {
+  const bool owned = valid && tx < kRtOX && ty < kRtOY;
+  const int l0 = tx + kRtBX * ty;
+  if (valid) {
+    float xo[3], inc[3];
+    cell_update_codes<MODEL>(k, X.f, MASKED ? codes : nullptr, i, j, xo, inc);
+    if (owned) {
 #pragma unroll
-        for (int a = 0; a < M::NV; ++a) Xo.f[a][ii + nx * jj] = xo[a];
-      }
-      s_u[li + kRtW * lj] = xo[0];
+      for (int a = 0; a < M::NV; ++a) Xo.f[a][i + nx * j] = xo[a];
     }
+    s_u[l0] = xo[0];
   }
   __syncthreads();
-  if (!tip) return;
+  if (!owned || !tip_inside(i, j, nx, ny, MASKED, g.r2)) return;
   // corners past the last column/row are the cell itself (tipTracker.cu:58-63)
   const bool ex = i < nx - 1, ey = j < ny - 1;
-  const int l0 = tx + kRtW * ty;
-  const int lx = ex ? l0 + 1 : l0, ly = ey ? l0 + kRtW : l0, lxy = (ex && ey) ? l0 + kRtW + 1 : l0;
+  const int lx = ex ? l0 + 1 : l0, ly = ey ? l0 + kRtBX : l0;
+  const int lxy = (ex && ey) ? l0 + kRtBX + 1 : l0;
   const int c = i + nx * j;
   const int sx = ex ? c + 1 : c, sy = ey ? c + nx : c, sxy = (ex && ey) ? c + nx + 1 : c;
   const float* pr = X.f[0];
@@ -669,7 +664,7 @@ static int sr_enqueue(yh_plan* pl, bool first, int slot, hipStream_t s) {
   const int model = pl->p.model;
   // 1 RD + tips
   {
-    dim3 rb(64, 4), rg((g.nx + 63) / 64, (g.ny + 3) / 4);
+    dim3 rb(kRtBX, kRtBY), rg((g.nx + kRtOX - 1) / kRtOX, (g.ny + kRtOY - 1) / kRtOY);
     const uint8_t* codes = pl->codes_d;
     const double Uth = pl->p.Uth;
     if (model == M2V) {
