@@ -85,6 +85,34 @@ def test_gpu_adjoint_matches_oracle(torch_cuda, masked):
         assert gt[k] == pytest.approx(fd, rel=1e-3), (name, gt[k], fd)
 
 
+@pytest.mark.parametrize("masked,n,k", [(False, 64, 8), (True, 64, 5), (True, 45, 8),
+                                         (False, 45, 3)])
+def test_tiled_steps_equal_one_step(torch_cuda, masked, n, k):
+    """The LDS-tiled forward and reverse (k steps per launch; forward launches
+    end on every checkpoint, 70 steps here, reverse launches on segment
+    starts) against the one-step kernels, on ragged tiles (45 = 32 + 13) and
+    with a mask: traces and loss bit for bit; the gradients to 1e-12, since
+    only the grouping of their FP64 sums over steps and workgroups differs
+    (lambda itself is computed by the same per-cell code)."""
+    torch = torch_cuda
+    p, x0, sensors, Dh, D, solid, nsteps, rec = _case(masked, n=n, nsteps=300, rec=3)
+    out = []
+    for kk in (1, k):
+        pr = SensorProblem(p, nsteps, sensors, x0, rec_every=rec, ckpt_every=70, mask=solid)
+        pr.set_substeps(kk)
+        _, tr = pr.forward(torch.from_numpy(Dh).cuda())
+        pr.set_targets(np.full_like(tr, 0.3))
+        L, _ = pr.forward(torch.from_numpy(D).cuda())
+        gD, gt = pr.backward()
+        out.append((tr, L, gD.cpu().numpy(), np.array(gt)))
+        pr.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
+    scale = np.max(np.abs(out[0][2]))
+    np.testing.assert_allclose(out[1][2], out[0][2], rtol=1e-12, atol=1e-12 * scale)
+    np.testing.assert_allclose(out[1][3], out[0][3], rtol=1e-12)
+
+
 def test_autograd_function(torch_cuda):
     torch = torch_cuda
     p, x0, sensors, Dh, D, solid, nsteps, rec = _case(False, n=48, nsteps=240, rec=4)

@@ -74,6 +74,7 @@ PROTOTYPES = {
     "yh_batch_autotune": (I, [P, I, P, C.POINTER(YhKernelCfg), C.POINTER(C.c_float)]),
     "yh_adjoint_create": (I, [PARAMS_P, I, I, I, P, I, C.POINTER(P)]),
     "yh_adjoint_destroy": (None, [P]),
+    "yh_adjoint_set_substeps": (I, [P, I]),
     "yh_adjoint_set_mask": (I, [P, P]),
     "yh_adjoint_set_initial": (I, [P, P, P, P]),
     "yh_adjoint_set_targets": (I, [P, P]),

@@ -46,6 +46,11 @@ class SensorProblem:
         self._last = None
         self._gen = 0
 
+    def set_substeps(self, k):
+        """Steps per launch of the forward and reverse sweeps (1: one-step
+        kernels; 2..8: LDS-tiled)."""
+        check(_protos().yh_adjoint_set_substeps(self._h, int(k)), "yh_adjoint_set_substeps")
+
     def close(self):
         if getattr(self, "_h", None):
             _protos().yh_adjoint_destroy(self._h)

@@ -36,6 +36,9 @@ struct AdjK {
   float dt, tc;
   float hw;        // Heaviside width
   float td, tr, tsi, t0, tvp, tv1m, tv2m, twp, twm, uc, uv, ucsi, kk;
+  // reciprocals, rounded once on the host from FP64: 1/hw, 0.5/hw, 1/tau_d,
+  // 1/tau_r, 0.5/tau_si, 1/tau_0, 1/tau_v+, 1/tau_w-, 1/tau_w+
+  float ihw, hihw, itd, itr, ihtsi, it0, itvp, itwm, itwp;
 };
 
 struct FkEval {
@@ -45,50 +48,55 @@ struct FkEval {
   float dGv_dtvp, dGw_dtwp;
 };
 
-__device__ __forceinline__ void heav(float x, float w, float* h, float* dh) {
-  const float t = tanhf(x / w);
+__device__ __forceinline__ void heav(float x, float iw, float hiw, float* h, float* dh) {
+#pragma clang fp contract(off)
+  const float t = tanhf(x * iw);
   *h = 0.5f * (1.0f + t);
-  *dh = 0.5f / w * (1.0f - t * t);
+  *dh = hiw * (1.0f - t * t);
 }
 
-// Fenton & Karma 1998 currents and derivatives (smoothed Heaviside).
+// Fenton & Karma 1998 currents and derivatives (smoothed Heaviside).  No
+// FMA contraction here or in adj_fwd_cell: every operation rounds on its
+// own, so the kernels that inline them agree bit for bit.
 __device__ FkEval fk_eval(const AdjK& k, float u, float v, float w) {
+#pragma clang fp contract(off)
   FkEval r;
   float p, dp, q, dq;
-  heav(u - k.uc, k.hw, &p, &dp);
-  heav(u - k.uv, k.hw, &q, &dq);
+  heav(u - k.uc, k.ihw, k.hihw, &p, &dp);
+  heav(u - k.uv, k.ihw, k.hihw, &q, &dq);
   const float f = (1.0f - u) * (u - k.uc);
   const float df = 1.0f + k.uc - 2.0f * u;
-  const float Jfi = -v * p * f / k.td;
-  const float Jso = u * (1.0f - p) / k.t0 + p / k.tr;
+  const float Jfi = -v * p * f * k.itd;
+  const float Jso = u * (1.0f - p) * k.it0 + p * k.itr;
   const float th = tanhf(k.kk * (u - k.ucsi));
-  const float Jsi = -w * (1.0f + th) / (2.0f * k.tsi);
+  const float Jsi = -w * (1.0f + th) * k.ihtsi;
   r.I = Jfi + Jso + Jsi;
-  r.dIdu = -v * (dp * f + p * df) / k.td + (1.0f - p) / k.t0 - u * dp / k.t0 + dp / k.tr -
-           w * k.kk * (1.0f - th * th) / (2.0f * k.tsi);
-  r.dIdv = -p * f / k.td;
-  r.dIdw = -(1.0f + th) / (2.0f * k.tsi);
+  r.dIdu = -v * (dp * f + p * df) * k.itd + (1.0f - p) * k.it0 - u * dp * k.it0 + dp * k.itr -
+           w * k.kk * (1.0f - th * th) * k.ihtsi;
+  r.dIdv = -p * f * k.itd;
+  r.dIdw = -(1.0f + th) * k.ihtsi;
   const float tvm = (1.0f - q) * k.tv1m + q * k.tv2m;
+  const float itvm = 1.0f / tvm;
   const float dtvm = dq * (k.tv2m - k.tv1m);
-  const float A = (1.0f - p) * (1.0f - v) / tvm - p * v / k.tvp;
-  const float dAdu = -dp * (1.0f - v) / tvm - (1.0f - p) * (1.0f - v) * dtvm / (tvm * tvm) -
-                     dp * v / k.tvp;
-  const float dAdv = -(1.0f - p) / tvm - p / k.tvp;
-  const float B = (1.0f - p) * (1.0f - w) / k.twm - p * w / k.twp;
-  const float dBdu = -dp * (1.0f - w) / k.twm - dp * w / k.twp;
-  const float dBdw = -(1.0f - p) / k.twm - p / k.twp;
+  const float A = (1.0f - p) * (1.0f - v) * itvm - p * v * k.itvp;
+  const float dAdu = -dp * (1.0f - v) * itvm - (1.0f - p) * (1.0f - v) * dtvm * (itvm * itvm) -
+                     dp * v * k.itvp;
+  const float dAdv = -(1.0f - p) * itvm - p * k.itvp;
+  const float B = (1.0f - p) * (1.0f - w) * k.itwm - p * w * k.itwp;
+  const float dBdu = -dp * (1.0f - w) * k.itwm - dp * w * k.itwp;
+  const float dBdw = -(1.0f - p) * k.itwm - p * k.itwp;
   r.Gv = -A;
   r.Gw = -B;
   r.dGvdu = -dAdu;
   r.dGvdv = -dAdv;
   r.dGwdu = -dBdu;
   r.dGwdw = -dBdw;
-  r.dI_dtau[0] = -Jfi / k.td;
-  r.dI_dtau[1] = -p / (k.tr * k.tr);
-  r.dI_dtau[2] = -Jsi / k.tsi;
-  r.dI_dtau[3] = -u * (1.0f - p) / (k.t0 * k.t0);
-  r.dGv_dtvp = -p * v / (k.tvp * k.tvp);
-  r.dGw_dtwp = -p * w / (k.twp * k.twp);
+  r.dI_dtau[0] = -Jfi * k.itd;
+  r.dI_dtau[1] = -p * (k.itr * k.itr);
+  r.dI_dtau[2] = -2.0f * Jsi * k.ihtsi;
+  r.dI_dtau[3] = -u * (1.0f - p) * (k.it0 * k.it0);
+  r.dGv_dtvp = -p * v * (k.itvp * k.itvp);
+  r.dGw_dtwp = -p * w * (k.itwp * k.itwp);
   return r;
 }
 
@@ -111,7 +119,38 @@ __device__ __forceinline__ Coef4 coef_of(const uint8_t* codes, int c) {
   return r;
 }
 
+// Mask coefficients of a stencil code (0xFF: hole).
+__device__ __forceinline__ Coef4 coef_from_code(uint32_t b) {
+  Coef4 r;
+  r.sc = b != kHole;
+  const uint32_t q = r.sc ? b : 0u;
+  r.W = (float)(q & 3u);
+  r.E = (float)((q >> 2) & 3u);
+  r.N = (float)((q >> 4) & 3u);
+  r.S = (float)((q >> 6) & 3u);
+  return r;
+}
+constexpr uint8_t kCodeAllOnes = 0x55;  // cW = cE = cN = cS = 1: no mask
+
 // ---------------------------------------------------------------- forward
+// One forward step of one cell from its state, its D, its four (mirror)
+// neighbours' u and D and its mask coefficients; shared by both forward
+// kernels, so they agree bit for bit.
+__device__ __forceinline__ void adj_fwd_cell(const AdjK& k, float u, float v, float w, float Dc,
+                                             float uW, float uE, float uN, float uS, float DW,
+                                             float DE, float DN, float DS, const Coef4& cf,
+                                             float& un, float& vn, float& wn) {
+#pragma clang fp contract(off)
+  const float lap = k.rxh * (cf.W * 0.5f * (Dc + DW) * (uW - u) +
+                             cf.E * 0.5f * (Dc + DE) * (uE - u)) +
+                    k.ryh * (cf.N * 0.5f * (Dc + DN) * (uN - u) +
+                             cf.S * 0.5f * (Dc + DS) * (uS - u));
+  const FkEval r = fk_eval(k, u, v, w);
+  un = cf.sc ? u + k.tc * (lap - k.dt * r.I) : 0.0f;
+  vn = cf.sc ? v + k.tc * (-k.dt * r.Gv) : 0.0f;
+  wn = cf.sc ? w + k.tc * (-k.dt * r.Gw) : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void k_adj_fwd(const AdjK k, CFields in, Fields out,
                                                  const float* __restrict__ D,
                                                  const uint8_t* __restrict__ codes,
@@ -126,26 +165,203 @@ __global__ __launch_bounds__(256) void k_adj_fwd(const AdjK k, CFields in, Field
   const int N = i + nx * mirror(j + 1, ny), S = i + nx * mirror(j - 1, ny);
   const Coef4 cf = coef_of(codes, c);
   const float* U = in.f[0];
-  const float u = U[c], v = in.f[1][c], w = in.f[2][c];
-  const float Dc = D[c];
-  const float lap = k.rxh * (cf.W * 0.5f * (Dc + D[W]) * (U[W] - u) +
-                             cf.E * 0.5f * (Dc + D[E]) * (U[E] - u)) +
-                    k.ryh * (cf.N * 0.5f * (Dc + D[N]) * (U[N] - u) +
-                             cf.S * 0.5f * (Dc + D[S]) * (U[S] - u));
-  const FkEval r = fk_eval(k, u, v, w);
-  const float un = cf.sc ? u + k.tc * (lap - k.dt * r.I) : 0.0f;
+  float un, vn, wn;
+  adj_fwd_cell(k, U[c], in.f[1][c], in.f[2][c], D[c], U[W], U[E], U[N], U[S], D[W], D[E], D[N],
+               D[S], cf, un, vn, wn);
   out.f[0][c] = un;
-  out.f[1][c] = cf.sc ? v + k.tc * (-k.dt * r.Gv) : 0.0f;
-  out.f[2][c] = cf.sc ? w + k.tc * (-k.dt * r.Gw) : 0.0f;
+  out.f[1][c] = vn;
+  out.f[2][c] = wn;
   if (trace_row) {
     const int s = smap[c];
     if (s >= 0) trace_row[s] = un;
   }
 }
 
+// K forward steps (K <= kAfKmax) of one kAfTX x kAfTY output tile through
+// LDS.  The tile's region is the output grown by K cells on each side and
+// clipped to the domain; mirror neighbours at the domain edge lie inside the
+// region, so in substep s only the cells s+1 away from region sides interior
+// to the domain lose an exact input and are skipped.  u is double-buffered
+// (neighbours read it); v and w are read by their own cell only and are
+// updated in place.  Outputs: with seg, the state after every substep into
+// seg slots slot0+1 .. slot0+K (the segment recompute); otherwise the final
+// state into out.  Sensor traces for every step with step % rec == 0.
+constexpr int kAfTX = 16, kAfTY = 16, kAfKmax = 8, kAfThreads = 256;
+constexpr int kAfRN = (kAfTX + 2 * kAfKmax) * (kAfTY + 2 * kAfKmax);
+
+__global__ __launch_bounds__(kAfThreads) void k_adj_fwd_tiled(
+    const AdjK k, CFields in, Fields out, const float* __restrict__ D,
+    const uint8_t* __restrict__ codes, const int* __restrict__ smap, int K, int tiles_x,
+    int step0, int rec, float* __restrict__ traces, int ns, float* __restrict__ seg, size_t n,
+    int slot0) {
+  __shared__ float s_u[2][kAfRN];
+  __shared__ float s_v[kAfRN], s_w[kAfRN], s_D[kAfRN];
+  __shared__ uint8_t s_c[kAfRN];
+  const int nx = k.nx, ny = k.ny;
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int ox0 = tx * kAfTX, oy0 = ty * kAfTY;
+  const int ox1 = min(ox0 + kAfTX, nx), oy1 = min(oy0 + kAfTY, ny);
+  const int gx0 = max(ox0 - K, 0), gy0 = max(oy0 - K, 0);
+  const int gx1 = min(ox1 + K, nx), gy1 = min(oy1 + K, ny);
+  const int rw = gx1 - gx0, rn = rw * (gy1 - gy0);
+  for (int l = threadIdx.x; l < rn; l += kAfThreads) {
+    const size_t c = (size_t)(gx0 + l % rw) + (size_t)nx * (gy0 + l / rw);
+    s_u[0][l] = in.f[0][c];
+    s_v[l] = in.f[1][c];
+    s_w[l] = in.f[2][c];
+    s_D[l] = D[c];
+    s_c[l] = codes ? codes[c] : kCodeAllOnes;
+  }
+  __syncthreads();
+  for (int s = 0; s < K; ++s) {
+    const int b = s & 1;
+    const int cx0 = gx0 > 0 ? gx0 + s + 1 : 0, cx1 = gx1 < nx ? gx1 - s - 1 : nx;
+    const int cy0 = gy0 > 0 ? gy0 + s + 1 : 0, cy1 = gy1 < ny ? gy1 - s - 1 : ny;
+    const int cw = cx1 - cx0, cn = cw * (cy1 - cy0);
+    const int step = step0 + s + 1;
+    float* row = (traces && step % rec == 0) ? traces + (size_t)(step / rec - 1) * ns : nullptr;
+    const float* U = s_u[b];
+    for (int q = threadIdx.x; q < cn; q += kAfThreads) {
+      const int gi = cx0 + q % cw, gj = cy0 + q / cw;
+      const int li = gi - gx0, lj = gj - gy0;
+      const int l = li + rw * lj;
+      const int lW = (mirror(gi - 1, nx) - gx0) + rw * lj;
+      const int lE = (mirror(gi + 1, nx) - gx0) + rw * lj;
+      const int lN = li + rw * (mirror(gj + 1, ny) - gy0);
+      const int lS = li + rw * (mirror(gj - 1, ny) - gy0);
+      float un, vn, wn;
+      adj_fwd_cell(k, U[l], s_v[l], s_w[l], s_D[l], U[lW], U[lE], U[lN], U[lS], s_D[lW],
+                   s_D[lE], s_D[lN], s_D[lS], coef_from_code(s_c[l]), un, vn, wn);
+      s_u[b ^ 1][l] = un;
+      s_v[l] = vn;
+      s_w[l] = wn;
+      if (gi >= ox0 && gi < ox1 && gj >= oy0 && gj < oy1) {
+        const size_t c = (size_t)gi + (size_t)nx * gj;
+        if (seg) {
+          float* base = seg + (size_t)(slot0 + s + 1) * 3 * n;
+          base[c] = un;
+          base[n + c] = vn;
+          base[2 * n + c] = wn;
+        } else if (s == K - 1) {
+          out.f[0][c] = un;
+          out.f[1][c] = vn;
+          out.f[2][c] = wn;
+        }
+        if (row) {
+          const int sm = smap[c];
+          if (sm >= 0) row[sm] = un;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+static hipError_t launch_adj_fwd_tiled(const AdjK& k, CFields in, Fields out, const float* D,
+                                       const uint8_t* codes, const int* smap, int K, int step0,
+                                       int rec, float* traces, int ns, float* seg, size_t n,
+                                       int slot0, hipStream_t s) {
+  if (K < 1 || K > kAfKmax) return hipErrorInvalidValue;
+  const int tiles_x = (k.nx + kAfTX - 1) / kAfTX, tiles_y = (k.ny + kAfTY - 1) / kAfTY;
+  hipLaunchKernelGGL(k_adj_fwd_tiled, dim3(tiles_x * tiles_y), dim3(kAfThreads), 0, s, k, in,
+                     out, D, codes, smap, K, tiles_x, step0, rec, traces, ns, seg, n, slot0);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- reverse
-// One reverse step: lam_out = J(X)^T lam_in (+ sensor source), gD and gtau
-// accumulated.  X is the state the forward step started from.
+// One reverse step of one cell (i, j): lam_out = J(X)^T lam_in (+ source),
+// its dL/dD term gd and its dL/dtau terms gt.  X is the state the forward
+// step started from.  The accessor gives u of X, D, lambda_u of lam_in and
+// the stencil code at offsets (di, dj) from the cell; every offset used is a
+// cell of the domain.  Shared by both reverse kernels (no contraction), so
+// they agree bit for bit on lambda and on each step's terms.
+template <class Acc>
+__device__ __forceinline__ void adj_rev_cell(const AdjK& k, const Acc& A, int i, int j, float v,
+                                             float w, float lv_in, float lw_in, float src,
+                                             float& lu, float& lv, float& lw, double& gd,
+                                             float gt[6]) {
+#pragma clang fp contract(off)
+  const int nx = k.nx, ny = k.ny;
+  const Coef4 cf = coef_from_code(A.code(0, 0));
+  const float u = A.U(0, 0);
+  const float m = cf.sc ? 1.0f : 0.0f;
+  const float mu = m * A.LU(0, 0), mv = m * lv_in, mw = m * lw_in;
+  const float Dc = A.D(0, 0);
+  const float tc = k.tc;
+  // own diffusion terms, neighbours W, E, N, S by mirror (helper_functions.cu:69-79)
+  const int ox[4] = {i > 0 ? -1 : 1, i < nx - 1 ? 1 : -1, 0, 0};
+  const int oy[4] = {0, 0, j < ny - 1 ? 1 : -1, j > 0 ? -1 : 1};
+  const float co[4] = {cf.W, cf.E, cf.N, cf.S};
+  const float rr[4] = {k.rxh, k.rxh, k.ryh, k.ryh};
+  float asum = 0.0f;
+  gd = 0.0;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) {
+    asum += rr[d] * co[d] * 0.5f * (Dc + A.D(ox[d], oy[d]));
+    gd += (double)(tc * mu * rr[d] * co[d] * 0.5f * (A.U(ox[d], oy[d]) - u));
+  }
+  // coefficients pointing at this cell from its neighbours (gather)
+  float lin = 0.0f;
+  auto from = [&](int di, int dj, float coef_c, float r) {
+    const float mc = A.LU(di, dj) * (coef_c != 0.0f ? 1.0f : 0.0f);  // coef 0 also covers holes
+    const float a = r * coef_c * 0.5f * (A.D(di, dj) + Dc);
+    lin += tc * mc * a;
+    gd += (double)(tc * mc * r * coef_c * 0.5f * (u - A.U(di, dj)));
+  };
+  if (i + 1 < nx) from(1, 0, coef_from_code(A.code(1, 0)).W, k.rxh);
+  if (i - 1 >= 0) from(-1, 0, coef_from_code(A.code(-1, 0)).E, k.rxh);
+  if (i == 1) from(-1, 0, coef_from_code(A.code(-1, 0)).W, k.rxh);
+  if (i == nx - 2) from(1, 0, coef_from_code(A.code(1, 0)).E, k.rxh);
+  if (j + 1 < ny) from(0, 1, coef_from_code(A.code(0, 1)).S, k.ryh);
+  if (j - 1 >= 0) from(0, -1, coef_from_code(A.code(0, -1)).N, k.ryh);
+  if (j == 1) from(0, -1, coef_from_code(A.code(0, -1)).S, k.ryh);
+  if (j == ny - 2) from(0, 1, coef_from_code(A.code(0, 1)).N, k.ryh);
+  const FkEval r = fk_eval(k, u, v, w);
+  const float tdt = tc * k.dt;
+  lu = mu * (1.0f - tc * asum - tdt * r.dIdu) + lin - mv * tdt * r.dGvdu - mw * tdt * r.dGwdu;
+  lv = -mu * tdt * r.dIdv + mv * (1.0f - tdt * r.dGvdv);
+  lw = -mu * tdt * r.dIdw + mw * (1.0f - tdt * r.dGwdw);
+  lu += src;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) gt[q] = mu * (-tdt * r.dI_dtau[q]);
+  gt[4] = mv * (-tdt * r.dGv_dtvp);
+  gt[5] = mw * (-tdt * r.dGw_dtwp);
+}
+
+// Accessor over device fields (row stride nx, cell index c).
+struct GlobalAcc {
+  const float *Uf, *Df, *LUf;
+  const uint8_t* codes;
+  int c, nx;
+  __device__ float U(int di, int dj) const { return Uf[c + di + nx * dj]; }
+  __device__ float D(int di, int dj) const { return Df[c + di + nx * dj]; }
+  __device__ float LU(int di, int dj) const { return LUf[c + di + nx * dj]; }
+  __device__ uint32_t code(int di, int dj) const {
+    return codes ? codes[c + di + nx * dj] : kCodeAllOnes;
+  }
+};
+
+// The sum of gt over a workgroup of NT threads (FP64, fixed order), added to
+// the workgroup's slot.
+template <int NT>
+__device__ __forceinline__ void adj_gtau_block(const double gt[6], double* slot) {
+  __shared__ double red[NT / 64][6];
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    double s = gt[q];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[wv][q] = s;
+  }
+  __syncthreads();
+  if (tid < 6) {
+    double s = 0.0;
+    for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
+    slot[tid] += s;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_adj_rev(const AdjK k, CFields X, CFields lam_in,
                                                  Fields lam_out, const float* __restrict__ D,
                                                  const uint8_t* __restrict__ codes,
@@ -155,80 +371,138 @@ __global__ __launch_bounds__(256) void k_adj_rev(const AdjK k, CFields X, CField
                                                  double* __restrict__ gtau_part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  float gt[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  double gtd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < k.nx && j < k.ny) {
-    const int nx = k.nx, ny = k.ny;
-    const int c = i + nx * j;
-    const float* U = X.f[0];
-    const float* LU = lam_in.f[0];
-    const Coef4 cf = coef_of(codes, c);
-    const float u = U[c], v = X.f[1][c], w = X.f[2][c];
-    const float m = cf.sc ? 1.0f : 0.0f;
-    const float mu = m * LU[c], mv = m * lam_in.f[1][c], mw = m * lam_in.f[2][c];
-    const float Dc = D[c];
-    const float tc = k.tc;
-    // own diffusion terms
-    const int nb[4] = {mirror(i - 1, nx) + nx * j, mirror(i + 1, nx) + nx * j,
-                       i + nx * mirror(j + 1, ny), i + nx * mirror(j - 1, ny)};
-    const float co[4] = {cf.W, cf.E, cf.N, cf.S};
-    const float rr[4] = {k.rxh, k.rxh, k.ryh, k.ryh};
-    float asum = 0.0f;
-    double gd = 0.0;
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-      asum += rr[d] * co[d] * 0.5f * (Dc + D[nb[d]]);
-      gd += (double)(tc * mu * rr[d] * co[d] * 0.5f * (U[nb[d]] - u));
-    }
-    // coefficients pointing at this cell from its neighbours (gather)
-    float lin = 0.0f;
-    auto from = [&](int cc, float coef_c, float r) {
-      const float mc = LU[cc] * (coef_c != 0.0f ? 1.0f : 0.0f);  // coef 0 also covers holes
-      const float a = r * coef_c * 0.5f * (D[cc] + Dc);
-      lin += tc * mc * a;
-      gd += (double)(tc * mc * r * coef_c * 0.5f * (u - U[cc]));
-    };
-    if (i + 1 < nx) { const int cc = c + 1; from(cc, coef_of(codes, cc).W, k.rxh); }
-    if (i - 1 >= 0) { const int cc = c - 1; from(cc, coef_of(codes, cc).E, k.rxh); }
-    if (i == 1) { const int cc = c - 1; from(cc, coef_of(codes, cc).W, k.rxh); }
-    if (i == nx - 2) { const int cc = c + 1; from(cc, coef_of(codes, cc).E, k.rxh); }
-    if (j + 1 < ny) { const int cc = c + nx; from(cc, coef_of(codes, cc).S, k.ryh); }
-    if (j - 1 >= 0) { const int cc = c - nx; from(cc, coef_of(codes, cc).N, k.ryh); }
-    if (j == 1) { const int cc = c - nx; from(cc, coef_of(codes, cc).S, k.ryh); }
-    if (j == ny - 2) { const int cc = c + nx; from(cc, coef_of(codes, cc).N, k.ryh); }
-    const FkEval r = fk_eval(k, u, v, w);
-    const float tdt = tc * k.dt;
-    float lu = mu * (1.0f - tc * asum - tdt * r.dIdu) + lin - mv * tdt * r.dGvdu -
-               mw * tdt * r.dGwdu;
-    const float lv = -mu * tdt * r.dIdv + mv * (1.0f - tdt * r.dGvdv);
-    const float lw = -mu * tdt * r.dIdw + mw * (1.0f - tdt * r.dGwdw);
+    const int c = i + k.nx * j;
+    GlobalAcc A{X.f[0], D, lam_in.f[0], codes, c, k.nx};
+    float src = 0.0f;
     if (src_row) {
       const int s = smap[c];
-      if (s >= 0) lu += src_row[s];
+      if (s >= 0) src = src_row[s];
     }
+    float lu, lv, lw, gt[6];
+    double gd;
+    adj_rev_cell(k, A, i, j, X.f[1][c], X.f[2][c], lam_in.f[1][c], lam_in.f[2][c], src, lu, lv,
+                 lw, gd, gt);
     lam_out.f[0][c] = lu;
     lam_out.f[1][c] = lv;
     lam_out.f[2][c] = lw;
     gD[c] += gd;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) gt[q] = mu * (-tdt * r.dI_dtau[q]);
-    gt[4] = mv * (-tdt * r.dGv_dtvp);
-    gt[5] = mw * (-tdt * r.dGw_dtwp);
+    for (int q = 0; q < 6; ++q) gtd[q] = (double)gt[q];
   }
-  // per-workgroup sum of the tau gradients, added to this workgroup's slot
-  __shared__ double red[4][6];
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
+  adj_gtau_block<256>(gtd, gtau_part + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * 6);
+}
+
+// Accessor over an LDS region of row stride rw (cell index l).
+struct LdsAcc {
+  const float *Uf, *Df, *LUf;
+  const uint8_t* codes;
+  int l, rw;
+  __device__ float U(int di, int dj) const { return Uf[l + di + rw * dj]; }
+  __device__ float D(int di, int dj) const { return Df[l + di + rw * dj]; }
+  __device__ float LU(int di, int dj) const { return LUf[l + di + rw * dj]; }
+  __device__ uint32_t code(int di, int dj) const { return codes[l + di + rw * dj]; }
+};
+
+// K reverse steps (K <= kArKmax), n_top, n_top-1, ..., of one kArTX x kArTY
+// output tile through LDS; the region is the output grown by K and clipped
+// to the domain, as in the tiled forward (lambda has the same one-cell
+// dependence).  u of each step's state is staged in LDS (neighbours read
+// it); v and w of the cell come from the segment buffer.  lambda_u is
+// double-buffered, lambda_v and lambda_w are read by their own cell only and
+// updated in place.  dL/dD of the output cells accumulates over the K steps
+// in LDS (FP64) and is added once; dL/dtau per thread, then per workgroup.
+constexpr int kArTX = 16, kArTY = 16, kArKmax = 8, kArThreads = 256;
+constexpr int kArRN = (kArTX + 2 * kArKmax) * (kArTY + 2 * kArKmax);
+
+__global__ __launch_bounds__(kArThreads) void k_adj_rev_tiled(
+    const AdjK k, const float* __restrict__ seg, size_t n, int slot_top, int n_top, CFields lam_in,
+    Fields lam_out, const float* __restrict__ D, const uint8_t* __restrict__ codes,
+    const int* __restrict__ smap, const float* __restrict__ src, int rec, int ns,
+    double* __restrict__ gD, double* __restrict__ gtau_part, int K, int tiles_x) {
+  __shared__ float s_U[kArRN], s_D[kArRN], s_lu[2][kArRN], s_lv[kArRN], s_lw[kArRN];
+  __shared__ uint8_t s_c[kArRN];
+  __shared__ double s_gd[kArTX * kArTY];
+  const int nx = k.nx, ny = k.ny;
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int ox0 = tx * kArTX, oy0 = ty * kArTY;
+  const int ox1 = min(ox0 + kArTX, nx), oy1 = min(oy0 + kArTY, ny);
+  const int gx0 = max(ox0 - K, 0), gy0 = max(oy0 - K, 0);
+  const int gx1 = min(ox1 + K, nx), gy1 = min(oy1 + K, ny);
+  const int rw = gx1 - gx0, rn = rw * (gy1 - gy0);
+  for (int l = threadIdx.x; l < rn; l += kArThreads) {
+    const size_t c = (size_t)(gx0 + l % rw) + (size_t)nx * (gy0 + l / rw);
+    s_D[l] = D[c];
+    s_c[l] = codes ? codes[c] : kCodeAllOnes;
+    s_lu[0][l] = lam_in.f[0][c];
+    s_lv[l] = lam_in.f[1][c];
+    s_lw[l] = lam_in.f[2][c];
+  }
+  for (int o = threadIdx.x; o < kArTX * kArTY; o += kArThreads) s_gd[o] = 0.0;
+  double gtd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < K; ++s) {
+    const int b = s & 1;
+    const int step = n_top - s;
+    const float* X = seg + (size_t)(slot_top - s) * 3 * n;
+    for (int l = threadIdx.x; l < rn; l += kArThreads)
+      s_U[l] = X[(size_t)(gx0 + l % rw) + (size_t)nx * (gy0 + l / rw)];
+    __syncthreads();
+    const float* srow = (src && step >= 1 && step % rec == 0) ? src + (size_t)(step / rec - 1) * ns
+                                                              : nullptr;
+    const int cx0 = gx0 > 0 ? gx0 + s + 1 : 0, cx1 = gx1 < nx ? gx1 - s - 1 : nx;
+    const int cy0 = gy0 > 0 ? gy0 + s + 1 : 0, cy1 = gy1 < ny ? gy1 - s - 1 : ny;
+    const int cw = cx1 - cx0, cn = cw * (cy1 - cy0);
+    for (int q = threadIdx.x; q < cn; q += kArThreads) {
+      const int gi = cx0 + q % cw, gj = cy0 + q / cw;
+      const int l = (gi - gx0) + rw * (gj - gy0);
+      const size_t c = (size_t)gi + (size_t)nx * gj;
+      LdsAcc A{s_U, s_D, s_lu[b], s_c, l, rw};
+      float sv = 0.0f;
+      if (srow) {
+        const int sm = smap[c];
+        if (sm >= 0) sv = srow[sm];
+      }
+      float lu, lv, lw, gt[6];
+      double gd;
+      adj_rev_cell(k, A, gi, gj, X[n + c], X[2 * n + c], s_lv[l], s_lw[l], sv, lu, lv, lw, gd,
+                   gt);
+      s_lu[b ^ 1][l] = lu;
+      s_lv[l] = lv;
+      s_lw[l] = lw;
+      if (gi >= ox0 && gi < ox1 && gj >= oy0 && gj < oy1) {
+        s_gd[(gi - ox0) + kArTX * (gj - oy0)] += gd;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    double s = (double)gt[q];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) red[wv][q] = s;
+        for (int t = 0; t < 6; ++t) gtd[t] += (double)gt[t];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  if (tid < 6) {
-    double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-    gtau_part[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * 6 + tid] += s;
+  const float* LU = s_lu[K & 1];
+  const int ow = ox1 - ox0, on = ow * (oy1 - oy0);
+  for (int q = threadIdx.x; q < on; q += kArThreads) {
+    const int gi = ox0 + q % ow, gj = oy0 + q / ow;
+    const int l = (gi - gx0) + rw * (gj - gy0);
+    const size_t c = (size_t)gi + (size_t)nx * gj;
+    lam_out.f[0][c] = LU[l];
+    lam_out.f[1][c] = s_lv[l];
+    lam_out.f[2][c] = s_lw[l];
+    gD[c] += s_gd[(gi - ox0) + kArTX * (gj - oy0)];
   }
+  adj_gtau_block<kArThreads>(gtd, gtau_part + (size_t)blockIdx.x * 6);
+}
+
+static hipError_t launch_adj_rev_tiled(const AdjK& k, const float* seg, size_t n, int slot_top,
+                                       int n_top, CFields li, Fields lo, const float* D,
+                                       const uint8_t* codes, const int* smap, const float* src,
+                                       int rec, int ns, double* gD, double* gtau_part, int K,
+                                       hipStream_t s) {
+  if (K < 1 || K > kArKmax) return hipErrorInvalidValue;
+  const int tiles_x = (k.nx + kArTX - 1) / kArTX, tiles_y = (k.ny + kArTY - 1) / kArTY;
+  hipLaunchKernelGGL(k_adj_rev_tiled, dim3(tiles_x * tiles_y), dim3(kArThreads), 0, s, k, seg, n,
+                     slot_top, n_top, li, lo, D, codes, smap, src, rec, ns, gD, gtau_part, K,
+                     tiles_x);
+  return hipGetLastError();
 }
 
 // lambda^N: zero, plus the sensor source of the last recorded step.
@@ -279,6 +553,8 @@ struct yh_adjoint {
   bool have_targets = false, forward_done = false;
   const float* D = nullptr;
   yh::AdjK k{};
+  int fwdK = 4;  // steps per launch, forward and reverse: 1 = one-step kernels, else LDS-tiled
+  int nparts = 0;  // dL/dtau workgroup slots: the larger of the two reverse grids
 };
 
 namespace yh {
@@ -323,6 +599,15 @@ static void adj_k(yh_adjoint* a, const double tau6[6]) {
   k.uv = (float)f.u_v;
   k.ucsi = (float)f.u_csi;
   k.kk = (float)f.k;
+  k.ihw = (float)(1.0 / p.heaviside_width);
+  k.hihw = (float)(0.5 / p.heaviside_width);
+  k.itd = (float)(1.0 / tau6[0]);
+  k.itr = (float)(1.0 / tau6[1]);
+  k.ihtsi = (float)(0.5 / tau6[2]);
+  k.it0 = (float)(1.0 / tau6[3]);
+  k.itvp = (float)(1.0 / tau6[4]);
+  k.itwp = (float)(1.0 / tau6[5]);
+  k.itwm = (float)(1.0 / f.tau_wm);
 }
 
 static inline float* slot(float* base, size_t n, int s, int var) {
@@ -374,6 +659,7 @@ int yh_adjoint_create(const yh_params* p, int nsteps, int rec_every, int ckpt_ev
   adj_k(a, tau6);
   dim3 bb(64, 4), bg((p->nx + 63) / 64, (p->ny + 3) / 4);
   a->nblocks = bg.x * bg.y;
+  a->nparts = std::max(a->nblocks, ((p->nx + kArTX - 1) / kArTX) * ((p->ny + kArTY - 1) / kArTY));
   hipError_t e = hipSuccess;
   auto M = [&](void** ptr, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(ptr, std::max<size_t>(bytes, 4));
@@ -391,7 +677,7 @@ int yh_adjoint_create(const yh_params* p, int nsteps, int rec_every, int ckpt_ev
   M((void**)&a->targets, (size_t)a->nrec * a->ns * 4);
   M((void**)&a->src, (size_t)a->nrec * a->ns * 4);
   M((void**)&a->gD, n * 8);
-  M((void**)&a->gtau_part, (size_t)a->nblocks * 6 * 8);
+  M((void**)&a->gtau_part, (size_t)a->nparts * 6 * 8);
   if (e == hipSuccess) e = hipMemcpy(a->smap, smap.data(), n * 4, hipMemcpyHostToDevice);
   for (int q = 0; q < 3 && e == hipSuccess; ++q) e = hipMemset(a->X0[q], 0, n * 4);
   if (e != hipSuccess) {
@@ -406,6 +692,12 @@ int yh_adjoint_create(const yh_params* p, int nsteps, int rec_every, int ckpt_ev
 void yh_adjoint_destroy(yh_adjoint* a) {
   if (a) (void)hipDeviceSynchronize();
   adj_free(a);
+}
+
+int yh_adjoint_set_substeps(yh_adjoint* a, int k) {
+  if (!a || k < 1 || k > kAfKmax || k > kArKmax) return YH_EINVAL;
+  a->fwdK = k;
+  return YH_OK;
 }
 
 int yh_adjoint_set_mask(yh_adjoint* a, const uint8_t* solid) {
@@ -454,19 +746,28 @@ int yh_adjoint_forward(yh_adjoint* a, const float* d_D, const double tau6[6], vo
   for (int q = 0; q < 3; ++q)
     YH_HIP(hipMemcpyAsync(a->X[0][q], a->X0[q], n * 4, hipMemcpyDeviceToDevice, s));
   int cur = 0;
-  for (int step = 1; step <= a->nsteps; ++step) {
-    if ((step - 1) % a->ckpt == 0) {  // checkpoint X^{step-1}
-      const int ci = (step - 1) / a->ckpt;
+  for (int done = 0; done < a->nsteps;) {
+    if (done % a->ckpt == 0) {  // checkpoint X^{done}
+      const int ci = done / a->ckpt;
       for (int q = 0; q < 3; ++q)
         YH_HIP(hipMemcpyAsync(slot(a->ck, n, ci, q), a->X[cur][q], n * 4,
                               hipMemcpyDeviceToDevice, s));
     }
-    float* row = (step % a->rec == 0) ? a->traces + (size_t)(step / a->rec - 1) * a->ns : nullptr;
     CFields in{{a->X[cur][0], a->X[cur][1], a->X[cur][2]}};
     Fields out{{a->X[cur ^ 1][0], a->X[cur ^ 1][1], a->X[cur ^ 1][2]}};
-    hipLaunchKernelGGL(k_adj_fwd, bg, bb, 0, s, a->k, in, out, d_D, a->codes, a->smap, row);
-    YH_HIP(hipGetLastError());
+    // launches end on every checkpoint step
+    const int K = std::min({a->fwdK, a->nsteps - done, (done / a->ckpt + 1) * a->ckpt - done});
+    if (a->fwdK == 1) {
+      const int step = done + 1;
+      float* row = (step % a->rec == 0) ? a->traces + (size_t)(step / a->rec - 1) * a->ns : nullptr;
+      hipLaunchKernelGGL(k_adj_fwd, bg, bb, 0, s, a->k, in, out, d_D, a->codes, a->smap, row);
+      YH_HIP(hipGetLastError());
+    } else {
+      YH_HIP(launch_adj_fwd_tiled(a->k, in, out, d_D, a->codes, a->smap, K, done, a->rec,
+                                  a->traces, a->ns, nullptr, n, 0, s));
+    }
     cur ^= 1;
+    done += K;
   }
   const size_t m = (size_t)a->nrec * a->ns;
   a->h_traces.resize(m);
@@ -497,7 +798,7 @@ int yh_adjoint_backward(yh_adjoint* a, void* stream, double* d_gD, double gtau[6
   const int N = a->nsteps;
   dim3 bb(64, 4), bg((a->p.nx + 63) / 64, (a->p.ny + 3) / 4);
   hipLaunchKernelGGL(k_fill_d, dim3(256), dim3(256), 0, s, a->gD, n, 0.0);
-  hipLaunchKernelGGL(k_fill_d, dim3(64), dim3(256), 0, s, a->gtau_part, (size_t)a->nblocks * 6, 0.0);
+  hipLaunchKernelGGL(k_fill_d, dim3(64), dim3(256), 0, s, a->gtau_part, (size_t)a->nparts * 6, 0.0);
   // lambda^N
   const float* srcN = (N % a->rec == 0) ? a->src + (size_t)(N / a->rec - 1) * a->ns : nullptr;
   int lc = 0;
@@ -511,33 +812,48 @@ int yh_adjoint_backward(yh_adjoint* a, void* stream, double* d_gD, double gtau[6
     for (int q = 0; q < 3; ++q)
       YH_HIP(hipMemcpyAsync(slot(a->seg, n, 0, q), slot(a->ck, n, seg, q), n * 4,
                             hipMemcpyDeviceToDevice, s));
-    for (int t = 1; t < s1 - s0; ++t) {
+    for (int t = 1; t < s1 - s0;) {
       CFields in{{slot(a->seg, n, t - 1, 0), slot(a->seg, n, t - 1, 1), slot(a->seg, n, t - 1, 2)}};
       Fields out{{slot(a->seg, n, t, 0), slot(a->seg, n, t, 1), slot(a->seg, n, t, 2)}};
-      hipLaunchKernelGGL(k_adj_fwd, bg, bb, 0, s, a->k, in, out, a->D, a->codes, a->smap,
-                         (float*)nullptr);
-      YH_HIP(hipGetLastError());
+      const int K = std::min(a->fwdK, s1 - s0 - t);
+      if (a->fwdK == 1) {
+        hipLaunchKernelGGL(k_adj_fwd, bg, bb, 0, s, a->k, in, out, a->D, a->codes, a->smap,
+                           (float*)nullptr);
+        YH_HIP(hipGetLastError());
+      } else {  // slots t .. t+K-1
+        YH_HIP(launch_adj_fwd_tiled(a->k, in, out, a->D, a->codes, a->smap, K, s0 + t - 1,
+                                    a->rec, nullptr, a->ns, a->seg, n, t - 1, s));
+      }
+      t += K;
     }
-    for (int step = s1 - 1; step >= s0; --step) {
+    for (int step = s1 - 1; step >= s0;) {
       const int t = step - s0;
-      CFields X{{slot(a->seg, n, t, 0), slot(a->seg, n, t, 1), slot(a->seg, n, t, 2)}};
       CFields li{{a->lam[lc][0], a->lam[lc][1], a->lam[lc][2]}};
       Fields lo{{a->lam[lc ^ 1][0], a->lam[lc ^ 1][1], a->lam[lc ^ 1][2]}};
-      const float* src = (step >= 1 && step % a->rec == 0)
-                             ? a->src + (size_t)(step / a->rec - 1) * a->ns
-                             : nullptr;
-      hipLaunchKernelGGL(k_adj_rev, bg, bb, 0, s, a->k, X, li, lo, a->D, a->codes, a->smap, src,
-                         a->gD, a->gtau_part);
-      YH_HIP(hipGetLastError());
+      if (a->fwdK == 1) {
+        CFields X{{slot(a->seg, n, t, 0), slot(a->seg, n, t, 1), slot(a->seg, n, t, 2)}};
+        const float* src = (step >= 1 && step % a->rec == 0)
+                               ? a->src + (size_t)(step / a->rec - 1) * a->ns
+                               : nullptr;
+        hipLaunchKernelGGL(k_adj_rev, bg, bb, 0, s, a->k, X, li, lo, a->D, a->codes, a->smap,
+                           src, a->gD, a->gtau_part);
+        YH_HIP(hipGetLastError());
+        step -= 1;
+      } else {  // steps step, step-1, ..., step-K+1 of this segment
+        const int K = std::min(a->fwdK, t + 1);
+        YH_HIP(launch_adj_rev_tiled(a->k, a->seg, n, t, step, li, lo, a->D, a->codes, a->smap,
+                                    a->src, a->rec, a->ns, a->gD, a->gtau_part, K, s));
+        step -= K;
+      }
       lc ^= 1;
     }
   }
   YH_HIP(hipMemcpyAsync(d_gD, a->gD, n * 8, hipMemcpyDeviceToDevice, s));
-  std::vector<double> part((size_t)a->nblocks * 6);
+  std::vector<double> part((size_t)a->nparts * 6);
   YH_HIP(hipMemcpyAsync(part.data(), a->gtau_part, part.size() * 8, hipMemcpyDeviceToHost, s));
   YH_HIP(hipStreamSynchronize(s));
   for (int q = 0; q < 6; ++q) gtau[q] = 0.0;
-  for (int b = 0; b < a->nblocks; ++b)
+  for (int b = 0; b < a->nparts; ++b)
     for (int q = 0; q < 6; ++q) gtau[q] += part[(size_t)b * 6 + q];
   return YH_OK;
 }
