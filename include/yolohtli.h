@@ -314,7 +314,7 @@ typedef struct yh_adjoint yh_adjoint;
 int yh_adjoint_create(const yh_params* p, int nsteps, int rec_every, int ckpt_every,
                       const int* sensors, int nsensors, yh_adjoint** out);
 void yh_adjoint_destroy(yh_adjoint* adj);
-/* Steps per launch of the forward and the reverse sweep, 1..8 (default 4):
+/* Steps per launch of the forward and the reverse sweep, 1..8 (default 8):
  * 1 runs one step per launch, more run that many steps of a tile through LDS
  * (the same per-cell arithmetic: forward and lambda bit for bit, the
  * gradients up to the grouping of their FP64 sums). */

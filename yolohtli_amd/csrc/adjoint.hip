@@ -48,9 +48,17 @@ struct FkEval {
   float dGv_dtvp, dGw_dtwp;
 };
 
+// tanh(x) = 1 - 2/(exp(2x) + 1) with the hardware exp2 and reciprocal:
+// absolute error ~1e-7, exact limits +-1 (exp2 saturates to inf or 0).
+__device__ __forceinline__ float tanh_fast(float x) {
+#pragma clang fp contract(off)
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2/ln 2
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
 __device__ __forceinline__ void heav(float x, float iw, float hiw, float* h, float* dh) {
 #pragma clang fp contract(off)
-  const float t = tanhf(x * iw);
+  const float t = tanh_fast(x * iw);
   *h = 0.5f * (1.0f + t);
   *dh = hiw * (1.0f - t * t);
 }
@@ -68,7 +76,7 @@ __device__ FkEval fk_eval(const AdjK& k, float u, float v, float w) {
   const float df = 1.0f + k.uc - 2.0f * u;
   const float Jfi = -v * p * f * k.itd;
   const float Jso = u * (1.0f - p) * k.it0 + p * k.itr;
-  const float th = tanhf(k.kk * (u - k.ucsi));
+  const float th = tanh_fast(k.kk * (u - k.ucsi));
   const float Jsi = -w * (1.0f + th) * k.ihtsi;
   r.I = Jfi + Jso + Jsi;
   r.dIdu = -v * (dp * f + p * df) * k.itd + (1.0f - p) * k.it0 - u * dp * k.it0 + dp * k.itr -
@@ -186,7 +194,7 @@ __global__ __launch_bounds__(256) void k_adj_fwd(const AdjK k, CFields in, Field
 // updated in place.  Outputs: with seg, the state after every substep into
 // seg slots slot0+1 .. slot0+K (the segment recompute); otherwise the final
 // state into out.  Sensor traces for every step with step % rec == 0.
-constexpr int kAfTX = 16, kAfTY = 16, kAfKmax = 8, kAfThreads = 256;
+constexpr int kAfTX = 16, kAfTY = 16, kAfKmax = 8, kAfThreads = 1024;
 constexpr int kAfRN = (kAfTX + 2 * kAfKmax) * (kAfTY + 2 * kAfKmax);
 
 __global__ __launch_bounds__(kAfThreads) void k_adj_fwd_tiled(
@@ -413,11 +421,12 @@ struct LdsAcc {
 // double-buffered, lambda_v and lambda_w are read by their own cell only and
 // updated in place.  dL/dD of the output cells accumulates over the K steps
 // in LDS (FP64) and is added once; dL/dtau per thread, then per workgroup.
-constexpr int kArTX = 16, kArTY = 16, kArKmax = 8, kArThreads = 256;
+constexpr int kArTX = 16, kArTY = 16, kArKmax = 8, kArThreads = 1024;
 constexpr int kArRN = (kArTX + 2 * kArKmax) * (kArTY + 2 * kArKmax);
 
 __global__ __launch_bounds__(kArThreads) void k_adj_rev_tiled(
-    const AdjK k, const float* __restrict__ seg, size_t n, int slot_top, int n_top, CFields lam_in,
+    const AdjK k, const float* __restrict__ seg, const float* __restrict__ x0, size_t n,
+    int slot_top, int n_top, CFields lam_in,
     Fields lam_out, const float* __restrict__ D, const uint8_t* __restrict__ codes,
     const int* __restrict__ smap, const float* __restrict__ src, int rec, int ns,
     double* __restrict__ gD, double* __restrict__ gtau_part, int K, int tiles_x) {
@@ -444,7 +453,8 @@ __global__ __launch_bounds__(kArThreads) void k_adj_rev_tiled(
   for (int s = 0; s < K; ++s) {
     const int b = s & 1;
     const int step = n_top - s;
-    const float* X = seg + (size_t)(slot_top - s) * 3 * n;
+    // the step's state: segment slot slot_top - s, slot 0 being the checkpoint x0
+    const float* X = slot_top - s == 0 ? x0 : seg + (size_t)(slot_top - s) * 3 * n;
     for (int l = threadIdx.x; l < rn; l += kArThreads)
       s_U[l] = X[(size_t)(gx0 + l % rw) + (size_t)nx * (gy0 + l / rw)];
     __syncthreads();
@@ -492,15 +502,16 @@ __global__ __launch_bounds__(kArThreads) void k_adj_rev_tiled(
   adj_gtau_block<kArThreads>(gtd, gtau_part + (size_t)blockIdx.x * 6);
 }
 
-static hipError_t launch_adj_rev_tiled(const AdjK& k, const float* seg, size_t n, int slot_top,
-                                       int n_top, CFields li, Fields lo, const float* D,
+static hipError_t launch_adj_rev_tiled(const AdjK& k, const float* seg, const float* x0,
+                                       size_t n, int slot_top, int n_top, CFields li, Fields lo,
+                                       const float* D,
                                        const uint8_t* codes, const int* smap, const float* src,
                                        int rec, int ns, double* gD, double* gtau_part, int K,
                                        hipStream_t s) {
   if (K < 1 || K > kArKmax) return hipErrorInvalidValue;
   const int tiles_x = (k.nx + kArTX - 1) / kArTX, tiles_y = (k.ny + kArTY - 1) / kArTY;
-  hipLaunchKernelGGL(k_adj_rev_tiled, dim3(tiles_x * tiles_y), dim3(kArThreads), 0, s, k, seg, n,
-                     slot_top, n_top, li, lo, D, codes, smap, src, rec, ns, gD, gtau_part, K,
+  hipLaunchKernelGGL(k_adj_rev_tiled, dim3(tiles_x * tiles_y), dim3(kArThreads), 0, s, k, seg, x0,
+                     n, slot_top, n_top, li, lo, D, codes, smap, src, rec, ns, gD, gtau_part, K,
                      tiles_x);
   return hipGetLastError();
 }
@@ -553,7 +564,7 @@ struct yh_adjoint {
   bool have_targets = false, forward_done = false;
   const float* D = nullptr;
   yh::AdjK k{};
-  int fwdK = 4;  // steps per launch, forward and reverse: 1 = one-step kernels, else LDS-tiled
+  int fwdK = 8;  // steps per launch, forward and reverse: 1 = one-step kernels, else LDS-tiled
   int nparts = 0;  // dL/dtau workgroup slots: the larger of the two reverse grids
 };
 
@@ -743,20 +754,20 @@ int yh_adjoint_forward(yh_adjoint* a, const float* d_D, const double tau6[6], vo
   a->D = d_D;
   const size_t n = a->n;
   dim3 bb(64, 4), bg((a->p.nx + 63) / 64, (a->p.ny + 3) / 4);
+  // checkpoint 0 = X^0; a launch that ends on a later checkpoint step writes
+  // its state into that checkpoint, which the next launch reads
   for (int q = 0; q < 3; ++q)
-    YH_HIP(hipMemcpyAsync(a->X[0][q], a->X0[q], n * 4, hipMemcpyDeviceToDevice, s));
-  int cur = 0;
+    YH_HIP(hipMemcpyAsync(slot(a->ck, n, 0, q), a->X0[q], n * 4, hipMemcpyDeviceToDevice, s));
+  CFields in{{slot(a->ck, n, 0, 0), slot(a->ck, n, 0, 1), slot(a->ck, n, 0, 2)}};
+  int nb = 0;  // the work buffer the next launch writes (never the one it reads)
   for (int done = 0; done < a->nsteps;) {
-    if (done % a->ckpt == 0) {  // checkpoint X^{done}
-      const int ci = done / a->ckpt;
-      for (int q = 0; q < 3; ++q)
-        YH_HIP(hipMemcpyAsync(slot(a->ck, n, ci, q), a->X[cur][q], n * 4,
-                              hipMemcpyDeviceToDevice, s));
-    }
-    CFields in{{a->X[cur][0], a->X[cur][1], a->X[cur][2]}};
-    Fields out{{a->X[cur ^ 1][0], a->X[cur ^ 1][1], a->X[cur ^ 1][2]}};
     // launches end on every checkpoint step
     const int K = std::min({a->fwdK, a->nsteps - done, (done / a->ckpt + 1) * a->ckpt - done});
+    const int end = done + K;
+    const bool to_ck = end % a->ckpt == 0 && end < a->nsteps;
+    const int ci = end / a->ckpt;
+    Fields out = to_ck ? Fields{{slot(a->ck, n, ci, 0), slot(a->ck, n, ci, 1), slot(a->ck, n, ci, 2)}}
+                       : Fields{{a->X[nb][0], a->X[nb][1], a->X[nb][2]}};
     if (a->fwdK == 1) {
       const int step = done + 1;
       float* row = (step % a->rec == 0) ? a->traces + (size_t)(step / a->rec - 1) * a->ns : nullptr;
@@ -766,8 +777,9 @@ int yh_adjoint_forward(yh_adjoint* a, const float* d_D, const double tau6[6], vo
       YH_HIP(launch_adj_fwd_tiled(a->k, in, out, d_D, a->codes, a->smap, K, done, a->rec,
                                   a->traces, a->ns, nullptr, n, 0, s));
     }
-    cur ^= 1;
-    done += K;
+    in = CFields{{out.f[0], out.f[1], out.f[2]}};
+    if (!to_ck) nb ^= 1;
+    done = end;
   }
   const size_t m = (size_t)a->nrec * a->ns;
   a->h_traces.resize(m);
@@ -808,12 +820,13 @@ int yh_adjoint_backward(yh_adjoint* a, void* stream, double* d_gD, double gtau[6
   for (int seg = a->nckpt - 1; seg >= 0; --seg) {
     const int s0 = seg * a->ckpt;
     const int s1 = std::min(N, s0 + a->ckpt);  // reverse steps n = s1-1 .. s0
-    // recompute X^{s0} .. X^{s1-1} into seg slots 0 .. s1-s0-1
-    for (int q = 0; q < 3; ++q)
-      YH_HIP(hipMemcpyAsync(slot(a->seg, n, 0, q), slot(a->ck, n, seg, q), n * 4,
-                            hipMemcpyDeviceToDevice, s));
+    // recompute X^{s0+1} .. X^{s1-1} into seg slots 1 .. s1-s0-1; X^{s0} is
+    // read from its checkpoint (slot 0 stands for it)
+    float* x0 = slot(a->ck, n, seg, 0);  // the checkpoint's three fields, n apart
     for (int t = 1; t < s1 - s0;) {
-      CFields in{{slot(a->seg, n, t - 1, 0), slot(a->seg, n, t - 1, 1), slot(a->seg, n, t - 1, 2)}};
+      CFields in = t == 1 ? CFields{{x0, x0 + n, x0 + 2 * n}}
+                          : CFields{{slot(a->seg, n, t - 1, 0), slot(a->seg, n, t - 1, 1),
+                                     slot(a->seg, n, t - 1, 2)}};
       Fields out{{slot(a->seg, n, t, 0), slot(a->seg, n, t, 1), slot(a->seg, n, t, 2)}};
       const int K = std::min(a->fwdK, s1 - s0 - t);
       if (a->fwdK == 1) {
@@ -831,7 +844,9 @@ int yh_adjoint_backward(yh_adjoint* a, void* stream, double* d_gD, double gtau[6
       CFields li{{a->lam[lc][0], a->lam[lc][1], a->lam[lc][2]}};
       Fields lo{{a->lam[lc ^ 1][0], a->lam[lc ^ 1][1], a->lam[lc ^ 1][2]}};
       if (a->fwdK == 1) {
-        CFields X{{slot(a->seg, n, t, 0), slot(a->seg, n, t, 1), slot(a->seg, n, t, 2)}};
+        CFields X = t == 0 ? CFields{{x0, x0 + n, x0 + 2 * n}}
+                           : CFields{{slot(a->seg, n, t, 0), slot(a->seg, n, t, 1),
+                                      slot(a->seg, n, t, 2)}};
         const float* src = (step >= 1 && step % a->rec == 0)
                                ? a->src + (size_t)(step / a->rec - 1) * a->ns
                                : nullptr;
@@ -841,8 +856,8 @@ int yh_adjoint_backward(yh_adjoint* a, void* stream, double* d_gD, double gtau[6
         step -= 1;
       } else {  // steps step, step-1, ..., step-K+1 of this segment
         const int K = std::min(a->fwdK, t + 1);
-        YH_HIP(launch_adj_rev_tiled(a->k, a->seg, n, t, step, li, lo, a->D, a->codes, a->smap,
-                                    a->src, a->rec, a->ns, a->gD, a->gtau_part, K, s));
+        YH_HIP(launch_adj_rev_tiled(a->k, a->seg, x0, n, t, step, li, lo, a->D, a->codes,
+                                    a->smap, a->src, a->rec, a->ns, a->gD, a->gtau_part, K, s));
         step -= K;
       }
       lc ^= 1;
