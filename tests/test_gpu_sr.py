@@ -103,3 +103,38 @@ def test_sr_singular_and_state_errors(torch_cuda):
     q.set_state(np.zeros(64 * 64, np.float32), np.zeros(64 * 64, np.float32))
     with pytest.raises(RuntimeError):
         q.sr_begin((32.0, 32.0))  # plan without reduceSym
+
+
+def test_sr_masked_matches_oracle(torch_cuda):
+    """The masked branch: holes in the tissue (zero state, mirrored no-flux
+    stencil, BFECC neighbour rules of advFDBFECC.cu:38-103) and the tip
+    search restricted to the centred disk (tipTracker.cu:47-51)."""
+    model = T.YH_MODEL_3V_FK
+    st, tip0 = _spiral_state(model)
+    mask = Y.make_holes(N, N, 6, 3.0, 5.0, Y.seed(9, 1, 1))
+    st = [np.where(mask != 0, x, 0.0).astype(np.float32) for x in st]
+    p = T.grid_params(N, N, model=model, reduceSym=1, tipOffsetX=R, tipOffsetY=R, solidSwitch=1)
+    pl = Y.Plan(p)
+    pl.set_mask(mask)
+    pl.set_state(*st)
+    pl.sr_begin(tip0)
+    hist = pl.sr_step(40, history=True)
+    info = pl.sr_info()
+    got = pl.get_state()
+    ref = O.SR(p, [x.astype(np.float64) for x in st], tip0=tip0, solid=mask)
+    want_hist = []
+    for _ in range(40):
+        want_hist.append(np.concatenate([ref.c, ref.phi]))
+        ref.step(1)
+    want_hist = np.array(want_hist)
+    print("cx,cy gpu", info["cx"], info["cy"], "oracle", ref.s.cx, ref.s.cy)
+    assert (info["cx"], info["cy"]) == (ref.s.cx, ref.s.cy)
+    scale = np.max(np.abs(want_hist), axis=0) + 1e-12
+    rel = np.max(np.abs(hist - want_hist) / scale, axis=0)
+    print("relative history error per component", rel)
+    assert np.all(rel < 1e-3), rel
+    for a in range(len(got)):
+        assert np.all(got[a][mask == 0] == 0)
+        err = np.max(np.abs(got[a] - ref.X[a]))
+        print("state", a, err)
+        assert err < 1e-4, (a, err)
