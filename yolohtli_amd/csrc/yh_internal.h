@@ -28,6 +28,9 @@ struct KParams {
   float u_c, u_v;
   float fk_A1, fk_A2, fk_A3, fk_A4;  // tcdt/tau_d, tcdt/tau_r, tcdt/tau_0, tcdt/tau_si
   float fk_b, fk_bu;                 // 2k*log2(e), 2k*log2(e)*u_csi
+  // J_si with tcdt/tau_si folded into the sigmoid: A4/(1 + 2^(bu - b u)) =
+  // 1/(2^(bs - b u) + 1/A4) with bs = bu - log2(A4)
+  float fk_bs, fk_iA4;
   float fk_B1, fk_B2, fk_B3;         // tcdt/tau_vp, tcdt/tau_v1m, tcdt/tau_v2m
   float fk_C1, fk_C2;                // tcdt/tau_wp, tcdt/tau_wm
   float hw;                          // tanh-smoothed Heaviside width (gradient path)
@@ -122,9 +125,9 @@ struct Model<MFK> {
     const float fi = __builtin_fmaf(s1 * s2, v, -k.fk_A2);           // -tcdt (J_fi + 1/tau_r)
     const float so = -k.fk_A3 * u;                                   // -tcdt u/tau_0
     const float r1 = p ? fi : so;
-    const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, k.fk_bu));
-    const float sg = __builtin_amdgcn_rcpf(ex + 1.0f);               // sigmoid(2k(u-u_csi))
-    du = r1 + (k.fk_A4 * w) * sg;
+    const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, k.fk_bs));
+    const float sg = __builtin_amdgcn_rcpf(ex + k.fk_iA4);           // A4 sigmoid(2k(u-u_csi))
+    du = __builtin_fmaf(w, sg, r1);
     const float bq = q ? k.fk_B3 : k.fk_B2;
     const float vo = __builtin_fmaf(-bq, v, bq);                     // (1-v) tcdt/tau_v-
     const float vc = -k.fk_B1 * v;
@@ -140,11 +143,11 @@ struct Model<MFK> {
   // reads at most one scalar register.  The empty asm makes the values opaque
   // so they stay in VGPRs instead of being re-materialised per cell.
   struct Pinned {
-    float bu, B2, B3;
+    float bs, B2, B3;
   };
   __device__ __forceinline__ static Pinned pin(const KParams& k) {
-    Pinned q{k.fk_bu, k.fk_B2, k.fk_B3};
-    asm volatile("" : "+v"(q.bu), "+v"(q.B2), "+v"(q.B3));
+    Pinned q{k.fk_bs, k.fk_B2, k.fk_B3};
+    asm volatile("" : "+v"(q.bs), "+v"(q.B2), "+v"(q.B3));
     return q;
   }
   // One whole explicit-Euler step of a cell, the same update as react():
@@ -164,9 +167,9 @@ struct Model<MFK> {
     const float fi = __builtin_fmaf(s1 * s2, v, -k.fk_A2);
     const float so = -k.fk_A3 * u;
     const float r1 = p ? fi : so;
-    const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, q.bu));
-    const float sg = __builtin_amdgcn_rcpf(ex + 1.0f);
-    u = ud + __builtin_fmaf(k.fk_A4 * w, sg, r1);
+    const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, q.bs));
+    const float sg = __builtin_amdgcn_rcpf(ex + k.fk_iA4);  // A4 sigmoid(2k(u-u_csi))
+    u = ud + __builtin_fmaf(w, sg, r1);
     const float bq = qv ? q.B3 : q.B2;
     const float vo = __builtin_fmaf(bq, 1.0f - v, v);
     const float vc = __builtin_fmaf(-k.fk_B1, v, v);
