@@ -170,13 +170,14 @@ struct Model<MFK> {
     const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, q.bs));
     const float sg = __builtin_amdgcn_rcpf(ex + k.fk_iA4);  // A4 sigmoid(2k(u-u_csi))
     u = ud + __builtin_fmaf(w, sg, r1);
+    // opening fma(b, 1 - v, v), closing fma(b, 0 - v, v) = fma(-b, v, v):
+    // one fma per gate with the rate and the target (1 or 0) selected
     const float bq = qv ? q.B3 : q.B2;
-    const float vo = __builtin_fmaf(bq, 1.0f - v, v);
-    const float vc = __builtin_fmaf(-k.fk_B1, v, v);
-    v = p ? vc : vo;
-    const float wo = __builtin_fmaf(k.fk_C2, 1.0f - w, w);
-    const float wc = __builtin_fmaf(-k.fk_C1, w, w);
-    w = p ? wc : wo;
+    const float tgt = p ? 0.0f : 1.0f;
+    const float gv = p ? k.fk_B1 : bq;
+    v = __builtin_fmaf(gv, tgt - v, v);
+    const float gw = p ? k.fk_C1 : k.fk_C2;
+    w = __builtin_fmaf(gw, tgt - w, w);
   }
 };
 
