@@ -64,6 +64,8 @@ void make_kparams(const yh_params& p, const yh_derived& d, KParams* k) {
   k->fk_bu = (float)(2.0 * f.k * log2e * f.u_csi);
   k->fk_bs = (float)(2.0 * f.k * log2e * f.u_csi - std::log2(tcdt / f.tau_si));
   k->fk_iA4 = (float)(f.tau_si / tcdt);
+  k->fk_P1 = (float)(tcdt / f.tau_d * (1.0 + f.u_c));
+  k->fk_P0 = (float)(-tcdt / f.tau_d * f.u_c);
   k->fk_B1 = (float)(tcdt / f.tau_vp);
   k->fk_B2 = (float)(tcdt / f.tau_v1m);
   k->fk_B3 = (float)(tcdt / f.tau_v2m);

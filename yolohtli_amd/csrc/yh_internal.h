@@ -31,6 +31,8 @@ struct KParams {
   // J_si with tcdt/tau_si folded into the sigmoid: A4/(1 + 2^(bu - b u)) =
   // 1/(2^(bs - b u) + 1/A4) with bs = bu - log2(A4)
   float fk_bs, fk_iA4;
+  // A1 (1-u)(u-u_c) = u (P1 - A1 u) + P0: P1 = A1 (1 + u_c), P0 = -A1 u_c
+  float fk_P1, fk_P0;
   float fk_B1, fk_B2, fk_B3;         // tcdt/tau_vp, tcdt/tau_v1m, tcdt/tau_v2m
   float fk_C1, fk_C2;                // tcdt/tau_wp, tcdt/tau_wm
   float hw;                          // tanh-smoothed Heaviside width (gradient path)
@@ -120,9 +122,8 @@ struct Model<MFK> {
     //   J_si = -w (1 + tanh(k(u-u_csi)))/(2 tau_si) = -w sigmoid(2k(u-u_csi))/tau_si
     const bool p = u >= k.u_c;
     const bool q = u >= k.u_v;
-    const float s1 = u - k.u_c;
-    const float s2 = __builtin_fmaf(u, -k.fk_A1, k.fk_A1);           // A1 (1-u)
-    const float fi = __builtin_fmaf(s1 * s2, v, -k.fk_A2);           // -tcdt (J_fi + 1/tau_r)
+    const float pa = __builtin_fmaf(u, __builtin_fmaf(u, -k.fk_A1, k.fk_P1), k.fk_P0);
+    const float fi = __builtin_fmaf(pa, v, -k.fk_A2);                // -tcdt (J_fi + 1/tau_r)
     const float so = -k.fk_A3 * u;                                   // -tcdt u/tau_0
     const float r1 = p ? fi : so;
     const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, k.fk_bs));
@@ -143,11 +144,11 @@ struct Model<MFK> {
   // reads at most one scalar register.  The empty asm makes the values opaque
   // so they stay in VGPRs instead of being re-materialised per cell.
   struct Pinned {
-    float bs, B2, B3;
+    float bs, B2, B3, P1;
   };
   __device__ __forceinline__ static Pinned pin(const KParams& k) {
-    Pinned q{k.fk_bs, k.fk_B2, k.fk_B3};
-    asm volatile("" : "+v"(q.bs), "+v"(q.B2), "+v"(q.B3));
+    Pinned q{k.fk_bs, k.fk_B2, k.fk_B3, k.fk_P1};
+    asm volatile("" : "+v"(q.bs), "+v"(q.B2), "+v"(q.B3), "+v"(q.P1));
     return q;
   }
   // One whole explicit-Euler step of a cell, the same update as react():
@@ -162,9 +163,8 @@ struct Model<MFK> {
                                               float& u, float& v, float& w) {
     const bool p = u >= k.u_c;
     const bool qv = u >= k.u_v;
-    const float s1 = u - k.u_c;
-    const float s2 = __builtin_fmaf(u, -k.fk_A1, k.fk_A1);
-    const float fi = __builtin_fmaf(s1 * s2, v, -k.fk_A2);
+    const float pa = __builtin_fmaf(u, __builtin_fmaf(u, -k.fk_A1, q.P1), k.fk_P0);
+    const float fi = __builtin_fmaf(pa, v, -k.fk_A2);
     const float so = -k.fk_A3 * u;
     const float r1 = p ? fi : so;
     const float ex = __builtin_amdgcn_exp2f(__builtin_fmaf(u, -k.fk_b, q.bs));
