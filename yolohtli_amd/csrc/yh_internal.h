@@ -143,12 +143,15 @@ struct Model<MFK> {
   // constants, an fma with two constant operands), and a VALU instruction
   // reads at most one scalar register.  The empty asm makes the values opaque
   // so they stay in VGPRs instead of being re-materialised per cell.
+  // C1, C2 also stop a select between two fields of a KParams held in
+  // memory (the batch table) from becoming a per-cell load of the selected
+  // field.
   struct Pinned {
-    float bs, B2, B3, P1;
+    float bs, B2, B3, P1, C1, C2;
   };
   __device__ __forceinline__ static Pinned pin(const KParams& k) {
-    Pinned q{k.fk_bs, k.fk_B2, k.fk_B3, k.fk_P1};
-    asm volatile("" : "+v"(q.bs), "+v"(q.B2), "+v"(q.B3), "+v"(q.P1));
+    Pinned q{k.fk_bs, k.fk_B2, k.fk_B3, k.fk_P1, k.fk_C1, k.fk_C2};
+    asm volatile("" : "+v"(q.bs), "+v"(q.B2), "+v"(q.B3), "+v"(q.P1), "+v"(q.C1), "+v"(q.C2));
     return q;
   }
   // One whole explicit-Euler step of a cell, the same update as react():
@@ -176,7 +179,7 @@ struct Model<MFK> {
     const float tgt = p ? 0.0f : 1.0f;
     const float gv = p ? k.fk_B1 : bq;
     v = __builtin_fmaf(gv, tgt - v, v);
-    const float gw = p ? k.fk_C1 : k.fk_C2;
+    const float gw = p ? q.C1 : q.C2;
     w = __builtin_fmaf(gw, tgt - w, w);
   }
 };
