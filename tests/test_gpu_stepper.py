@@ -53,6 +53,7 @@ def test_single_step_wrapper_matches_oracle(torch_cuda, model, masked):
 CFGS = [(K, r, w, c) for K in (1, 2, 3, 4, 8, 12)
         for (r, w) in ((4, 4), (8, 4), (16, 4), (4, 8), (8, 8), (16, 8), (4, 16), (8, 16))
         for c in (1, 2) if 2 * K < r * w and (c == 1 or (r * w <= 64 and r <= 8))]
+CFGS += [(K, 7, 16, 1) for K in (1, 3, 8, 12)]  # 7 rows per wave: not a power of two
 
 
 @pytest.mark.parametrize("model", MODELS)

@@ -156,7 +156,8 @@ __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, 
 bool tiled_supported(int rows, int waves, int cols) {
   const bool rw = (rows == 4 && (waves == 4 || waves == 8 || waves == 16)) ||
                   (rows == 8 && (waves == 4 || waves == 8 || waves == 16)) ||
-                  (rows == 16 && (waves == 4 || waves == 8));
+                  (rows == 16 && (waves == 4 || waves == 8)) ||
+                  (rows == 7 && waves == 16 && cols == 1);  // 64 x 112: 242 tiles at 1024^2
   // two columns per lane double the registers: 16 rows, or 8 rows in 16 waves, spill
   return rw && (cols == 1 || (cols == 2 && rows * waves <= 64 && rows <= 8));
 }
@@ -166,9 +167,9 @@ bool tiled_apd_supported(int rows, int waves, int cols) {
   return cols == 1 && (rows == 4 || rows == 8) && (waves == 8 || waves == 16);
 }
 
-// Shapes built with the previous-u output (the same set).
+// Shapes built with the previous-u output (the APD set and 7 x 16).
 bool tiled_prev_supported(int rows, int waves, int cols) {
-  return tiled_apd_supported(rows, waves, cols);
+  return tiled_apd_supported(rows, waves, cols) || (cols == 1 && rows == 7 && waves == 16);
 }
 
 template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV = false>
@@ -210,6 +211,7 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
     YH_TILE_PREV(8, 8)
     YH_TILE_PREV(4, 16)
     YH_TILE_PREV(8, 16)
+    YH_TILE_PREV(7, 16)
 #undef YH_TILE_PREV
     return hipErrorInvalidValue;
   }
@@ -224,6 +226,7 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
   YH_TILE_CASE(16, 8, 1)
   YH_TILE_CASE(4, 16, 1)
   YH_TILE_CASE(8, 16, 1)
+  YH_TILE_CASE(7, 16, 1)
   YH_TILE_CASE(4, 4, 2)
   YH_TILE_CASE(8, 4, 2)
   YH_TILE_CASE(4, 8, 2)
