@@ -29,10 +29,12 @@ def _batch(params, masks, fields, **cfg):
     return b
 
 
+@pytest.mark.parametrize("shape", [(6, 4, 16, 1), (8, 7, 16, 1)])
 @pytest.mark.parametrize("model", [T.YH_MODEL_2V_REF, T.YH_MODEL_3V_FK])
-def test_batch_equals_single_tissue_kernel(torch_cuda, model):
+def test_batch_equals_single_tissue_kernel(torch_cuda, model, shape):
     params, masks, fields = _small_workload(model)
-    cfg = dict(substeps=6, rows=4, waves=16, cols=1)
+    K, r, w, c = shape
+    cfg = dict(substeps=K, rows=r, waves=w, cols=c)
     b = _batch(params, masks, fields, persistent=1, **cfg)
     b.set_graph(False)
     b.step(27)
@@ -41,7 +43,7 @@ def test_batch_equals_single_tissue_kernel(torch_cuda, model):
         pl.set_mask(masks[t])
         pl.set_state(*fields[t])
         pl.set_option(T.YH_OPT_KEEP_PREV, 0)
-        pl.set_kernel(T.YH_KERNEL_TILED, 6, 4, 16, 1)
+        pl.set_kernel(T.YH_KERNEL_TILED, K, r, w, c)
         pl.step(27)
         want = pl.get_state()
         got = b.get_state(t)

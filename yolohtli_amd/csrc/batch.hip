@@ -104,13 +104,15 @@ static int max_resident(int* out) {
   YH_BATCH_ONE(M, MK, 4, 16, 1, FN, __VA_ARGS__)  \
   YH_BATCH_ONE(M, MK, 8, 16, 1, FN, __VA_ARGS__)  \
   YH_BATCH_ONE(M, MK, 16, 8, 1, FN, __VA_ARGS__)  \
+  YH_BATCH_ONE(M, MK, 7, 16, 1, FN, __VA_ARGS__)  \
   YH_BATCH_ONE(M, MK, 4, 8, 2, FN, __VA_ARGS__)   \
   YH_BATCH_ONE(M, MK, 8, 8, 2, FN, __VA_ARGS__)   \
   YH_BATCH_ONE(M, MK, 4, 16, 2, FN, __VA_ARGS__)
 
 static bool batch_shape_ok(int rows, int waves, int cols) {
   return (cols == 1 && ((rows == 4 && (waves == 8 || waves == 16)) ||
-                        (rows == 8 && (waves == 8 || waves == 16)) || (rows == 16 && waves == 8))) ||
+                        (rows == 8 && (waves == 8 || waves == 16)) || (rows == 16 && waves == 8) ||
+                        (rows == 7 && waves == 16))) ||
          (cols == 2 && ((rows == 4 && (waves == 8 || waves == 16)) || (rows == 8 && waves == 8)));
 }
 
@@ -485,7 +487,8 @@ int yh_batch_autotune(yh_batch* b, int steps, void* stream, yh_kernel_cfg* best,
   YH_HIP(hipEventCreate(&e1));
   int minside = 1 << 30;
   for (int t = 0; t < b->nt; ++t) minside = std::min(minside, std::min(b->p[t].nx, b->p[t].ny));
-  const int shapes[][3] = {{4, 8, 1}, {8, 8, 1}, {4, 16, 1}, {8, 16, 1}, {4, 8, 2}, {4, 16, 2}};
+  const int shapes[][3] = {{4, 8, 1}, {8, 8, 1}, {4, 16, 1}, {8, 16, 1}, {7, 16, 1},
+                           {4, 8, 2}, {4, 16, 2}};
   const int Ks[] = {2, 4, 6, 8, 12};
   yh_kernel_cfg bestc = cfg0;
   int bestp = pers0;
