@@ -53,7 +53,7 @@ def test_single_step_wrapper_matches_oracle(torch_cuda, model, masked):
 CFGS = [(K, r, w, c) for K in (1, 2, 3, 4, 8, 12)
         for (r, w) in ((4, 4), (8, 4), (16, 4), (4, 8), (8, 8), (16, 8), (4, 16), (8, 16))
         for c in (1, 2) if 2 * K < r * w and (c == 1 or (r * w <= 64 and r <= 8))]
-CFGS += [(K, 7, 16, 1) for K in (1, 3, 8, 12)]  # 7 rows per wave: not a power of two
+CFGS += [(K, 7, 16, 1) for K in (1, 3, 8, 9, 12)]  # 7 rows per wave: not a power of two
 
 
 @pytest.mark.parametrize("model", MODELS)
@@ -171,11 +171,13 @@ def test_tips_match_oracle_and_plan_log(torch_cuda):
 
 
 @pytest.mark.parametrize("masked", [False, True])
-@pytest.mark.parametrize("cfg", [(T.YH_KERNEL_TILED, 8, 8, 8, 1), (T.YH_KERNEL_TILED, 8, 8, 4, 1)])
+@pytest.mark.parametrize("cfg", [(T.YH_KERNEL_TILED, 8, 8, 8, 1), (T.YH_KERNEL_TILED, 8, 8, 4, 1),
+                                 (T.YH_KERNEL_TILED, 9, 7, 16, 1)])
 def test_tips_after_multistep_call_match_oracle(torch_cuda, cfg, masked):
-    """Tips after a call of 37 steps with K = 8 per launch.  The previous u
-    comes from the last launch (8 rows x 8 waves: it writes u one substep
-    before its end) or from a single-step tail (8 rows x 4 waves).  A second
+    """Tips after a call of 37 steps with K = 8 or 9 per launch.  The
+    previous u comes from the last launch (8 rows x 8 waves and 7 x 16: it
+    writes u one substep before its end; with K = 9 that launch is the
+    one-substep 37th step) or from a single-step tail (8 rows x 4 waves).  A second
     plan takes 36 steps without keeping the previous state, then 1, and
     exposes steps 3036 and 3037: the first plan's final state equals the
     second's bit for bit, and its tips are the oracle's tips between those

@@ -21,7 +21,7 @@ std::vector<yh_kernel_cfg> candidates(const yh_plan* pl) {
   std::vector<yh_kernel_cfg> c;
   c.push_back({YH_KERNEL_DIRECT, 1, 0, 0, 1});
   const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
-  const int Ks[] = {1, 2, 4, 6, 8, 12};
+  const int Ks[] = {1, 2, 4, 6, 8, 9, 10, 12};
   const int shapes[][3] = {{4, 8, 1}, {8, 8, 1}, {4, 16, 1}, {8, 16, 1}, {16, 8, 1},
                            {7, 16, 1}, {4, 8, 2}, {8, 8, 2}, {4, 16, 2}};
   for (int K : Ks) {
