@@ -119,6 +119,7 @@ static bool batch_shape_ok(int rows, int waves, int cols) {
 static hipError_t launch_batch(int model, bool masked, int rows, int waves, int cols, int grid,
                                const BatchTissue* t, const int2* items, int nitems, CFields in,
                                Fields out, const uint8_t* codes, int K, hipStream_t s) {
+  if (K < 1) return hipErrorInvalidValue;
   YH_BATCH_DISPATCH(launch_batch_t, grid, t, items, nitems, in, out, codes, K, s);
   return hipErrorInvalidValue;
 }

@@ -177,7 +177,7 @@ static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out
                                  const uint8_t* codes, hipStream_t s, const ApdState& apd,
                                  float* prev = nullptr) {
   const int wout = 64 * CW - 2 * K, hout = NW * R - 2 * K;
-  if (wout <= 0 || hout <= 0) return hipErrorInvalidValue;
+  if (K < 1 || wout <= 0 || hout <= 0) return hipErrorInvalidValue;
   const int tiles_x = (k.nx + wout - 1) / wout;
   const int tiles_y = (k.ny + hout - 1) / hout;
   const int ntiles = tiles_x * tiles_y;

@@ -3,6 +3,8 @@
 // kernel (batch.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "yh_internal.h"
 
 namespace yh {
@@ -18,6 +20,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int ntiles) {
   const int q = ntiles >> 3, rem = ntiles & 7;
   const int x = bid & 7, idx = bid >> 3;
   return x * q + (x < rem ? x : rem) + idx;
+}
+
+// Element idx of an array of at most 2^30 elements (the entry points reject
+// larger grids) as a 32-bit byte offset from the array base, so loads and
+// stores take the scalar-base + 32-bit vector-offset form instead of a
+// 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ T* at32(T* base, uint32_t idx) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T*>(reinterpret_cast<B*>(base) + (size_t)(idx * (uint32_t)sizeof(T)));
 }
 
 // Region of one workgroup: (64*CW) columns x (NW*R) rows; lane L owns columns
@@ -38,6 +50,7 @@ struct TileLds {
 // the measurement costs no extra launch or state traffic.  With PREV, u of
 // the output cells one substep before the end is written to prev_u (0 in
 // holes): the previous state the tip test needs, without a single-step tail.
+// K >= 1 (the launchers reject K < 1).
 template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false, bool PREV = false>
 __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Fields out,
                                              const uint8_t* __restrict__ codes, int K, int tx,
@@ -78,18 +91,24 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
   uint32_t hole[CW];    // bit r: cell (r, c) is a hole
 #pragma unroll
   for (int c = 0; c < CW; ++c) hole[c] = 0;
+  // Rows are loaded first, last, then 1 .. R-2: the row exchange of the
+  // first substep needs only the first and the last row, and row r of that
+  // substep only rows up to r+1, so (with the first substep peeled out of
+  // the loop below) each wave starts computing while its later rows are
+  // still arriving.
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int ri = 0; ri < R; ++ri) {
+    const int r = ri == 0 ? 0 : ri == 1 ? R - 1 : ri - 1;
     const int gy = gy0 + r;
     const int sy = reflect_clamp(gy, ny);
     const bool flip_y = (gy < 0) || (gy >= ny);
 #pragma unroll
     for (int c = 0; c < CW; ++c) {
-      const size_t idx = (size_t)sy * nx + sx[c];
+      const uint32_t idx = (uint32_t)sy * (uint32_t)nx + (uint32_t)sx[c];
 #pragma unroll
-      for (int a = 0; a < NV; ++a) X[a][r][c] = in.f[a][idx];
+      for (int a = 0; a < NV; ++a) X[a][r][c] = *at32(in.f[a], idx);
       if (MASKED) {
-        const uint32_t b = codes[idx];
+        const uint32_t b = *at32(codes, idx);
         const bool h = b == kHole;
         const uint32_t q = h ? 0u : b;
         uint32_t w2 = q & 3u, e2 = (q >> 2) & 3u, n2 = (q >> 4) & 3u, s2 = (q >> 6) & 3u;
@@ -135,7 +154,9 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
   [[maybe_unused]] const typename M::Pinned pinned = M::pin(k);
   const float dscale = k.iso ? k.trx : 1.0f;  // the laplacian terms below carry trx otherwise
 
-  for (int s = 0; s < K; ++s) {
+  // One substep; called once with s = 0 (peeled, so the loads above are
+  // waited for row by row) and then in a loop.
+  auto substep = [&](const int s) __attribute__((always_inline)) {
     const int b = s & 1;
 #pragma unroll
     for (int a = 0; a < ND; ++a)
@@ -225,15 +246,23 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
                      a_front[APD ? r : 0][c], a_back[APD ? r : 0][c], a_d1[APD ? r : 0][c],
                      a_d2[APD ? r : 0][c], a_first[APD ? r : 0][c]);
           }
-          if (PREV && s == K - 1) {
-            // u of an output cell one substep before the end (the rows
-            // active in the last substep are exactly the output rows)
+          if ((PREV || !APD) && s == K - 1) {
+            // The rows active in the last substep are exactly the output
+            // rows: store each as soon as it is final (overlapping the
+            // stores with the rest of the substep), and with PREV also u
+            // one substep before the end.
             int px = gx0 + c, py = gy0 + r;
             asm volatile("" : "+v"(px), "+s"(py));  // addresses computed here, not hoisted
             const int lc = lane * CW + c;
             if (lc >= K && lc < WR - K && px < nx && py < ny) {
               const bool h = MASKED && ((hole[c] >> r) & 1u);
-              prev_u[(size_t)py * nx + px] = h ? 0.0f : u_old;
+              const uint32_t idx = (uint32_t)py * (uint32_t)nx + (uint32_t)px;
+              if (PREV) *at32(prev_u, idx) = h ? 0.0f : u_old;
+              if (!APD) {
+#pragma unroll
+                for (int a = 0; a < NV; ++a)
+                  __builtin_nontemporal_store(h ? 0.0f : X[a][r][c], at32(out.f[a], idx));
+              }
             }
           }
         }
@@ -243,7 +272,10 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
 #pragma unroll
         for (int c = 0; c < CW; ++c) prev[a][c] = old[a][c];
     }
-  }
+  };
+  substep(0);
+  for (int s = 1; s < K; ++s) substep(s);
+  if (!APD) return;
 
 #pragma unroll
   for (int r = 0; r < R; ++r) {
