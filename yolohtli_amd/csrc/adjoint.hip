@@ -714,10 +714,8 @@ int yh_adjoint_set_substeps(yh_adjoint* a, int k) {
 int yh_adjoint_set_mask(yh_adjoint* a, const uint8_t* solid) {
   if (!a || !solid) return YH_EINVAL;
   if (!a->p.solidSwitch) return YH_ESTATE;
-  if (!a->solid) {
-    YH_HIP(hipMalloc(&a->solid, a->n));
-    YH_HIP(hipMalloc(&a->codes, a->n));
-  }
+  if (!a->solid) YH_HIP(hipMalloc(&a->solid, a->n));
+  if (!a->codes) YH_HIP(hipMalloc(&a->codes, a->n));
   std::vector<uint8_t> m(a->n);
   for (size_t c = 0; c < a->n; ++c) m[c] = solid[c] ? 1 : 0;
   YH_HIP(hipMemcpy(a->solid, m.data(), a->n, hipMemcpyHostToDevice));

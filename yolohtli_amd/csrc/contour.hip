@@ -121,12 +121,11 @@ int yh_contour(yh_plan* pl, int mode, double th1, double th2, double th3, const 
   if (mode == 1 && !pl->apd_on) return YH_ESTATE;  // mode 1 traces the APD alternans sign
   const size_t cells = pl->n;
   const int pcap = (int)std::min<size_t>(2 * cells, (size_t)1 << 30);
-  if (!pl->cont_pts) {
-    YH_HIP(hipMalloc(&pl->cont_pts, sizeof(float) * 3 * pcap));
-    YH_HIP(hipMalloc(&pl->cont_keys, sizeof(int) * pcap));
-    YH_HIP(hipMalloc(&pl->cont_count, sizeof(int)));
-    YH_HIP(hipMalloc(&pl->cont_area, cells));
-  }
+  // each buffer on its own, so a call after a failed allocation completes it
+  if (!pl->cont_pts) YH_HIP(hipMalloc(&pl->cont_pts, sizeof(float) * 3 * pcap));
+  if (!pl->cont_keys) YH_HIP(hipMalloc(&pl->cont_keys, sizeof(int) * pcap));
+  if (!pl->cont_count) YH_HIP(hipMalloc(&pl->cont_count, sizeof(int)));
+  if (!pl->cont_area) YH_HIP(hipMalloc(&pl->cont_area, cells));
   const uint8_t* darea = nullptr;
   if (area) {
     YH_HIP(hipMemcpy(pl->cont_area, area, cells, hipMemcpyHostToDevice));

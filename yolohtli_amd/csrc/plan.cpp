@@ -48,8 +48,9 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
   int64_t done = pl->count;
   if (plan_general(pl)) {
     const int order = pl->p.timeIntOrder;
-    if (order > 1 && !pl->gscratch[0]) {
-      for (int q = 0; q < 9; ++q) YH_HIP(hipMalloc(&pl->gscratch[q], pl->n * sizeof(float)));
+    if (order > 1) {  // completes an allocation an earlier failure left partial
+      for (int q = 0; q < 9; ++q)
+        if (!pl->gscratch[q]) YH_HIP(hipMalloc(&pl->gscratch[q], pl->n * sizeof(float)));
     }
     const GParams g = make_gparams(pl->p, pl->stim_on, pl->stim_x, pl->stim_y);
     for (int q = 0; q < nsteps; ++q) {
@@ -185,10 +186,8 @@ int yh_plan_params(const yh_plan* pl, yh_params* p, yh_derived* d) {
 
 int yh_set_mask(yh_plan* pl, const uint8_t* solid) {
   if (!pl || !solid) return YH_EINVAL;
-  if (!pl->solid_d) {
-    YH_HIP(hipMalloc(&pl->solid_d, pl->n));
-    YH_HIP(hipMalloc(&pl->codes_d, pl->n));
-  }
+  if (!pl->solid_d) YH_HIP(hipMalloc(&pl->solid_d, pl->n));
+  if (!pl->codes_d) YH_HIP(hipMalloc(&pl->codes_d, pl->n));
   std::vector<uint8_t> m(pl->n);
   for (size_t c = 0; c < pl->n; ++c) m[c] = solid[c] ? 1 : 0;
   YH_HIP(hipMemcpy(pl->solid_d, m.data(), pl->n, hipMemcpyHostToDevice));
@@ -322,13 +321,12 @@ int yh_apd_enable(yh_plan* pl, int on, double threshold) {
     pl->apd_on = false;
     return YH_OK;
   }
-  if (!pl->apd.front) {
-    YH_HIP(hipMalloc(&pl->apd.front, pl->n * sizeof(float)));
-    YH_HIP(hipMalloc(&pl->apd.back, pl->n * sizeof(float)));
-    YH_HIP(hipMalloc(&pl->apd.apd1, pl->n * sizeof(float)));
-    YH_HIP(hipMalloc(&pl->apd.apd2, pl->n * sizeof(float)));
-    YH_HIP(hipMalloc(&pl->apd.first, pl->n));
-  }
+  // each buffer on its own, so a call after a failed allocation completes it
+  if (!pl->apd.front) YH_HIP(hipMalloc(&pl->apd.front, pl->n * sizeof(float)));
+  if (!pl->apd.back) YH_HIP(hipMalloc(&pl->apd.back, pl->n * sizeof(float)));
+  if (!pl->apd.apd1) YH_HIP(hipMalloc(&pl->apd.apd1, pl->n * sizeof(float)));
+  if (!pl->apd.apd2) YH_HIP(hipMalloc(&pl->apd.apd2, pl->n * sizeof(float)));
+  if (!pl->apd.first) YH_HIP(hipMalloc(&pl->apd.first, pl->n));
   YH_HIP(hipDeviceSynchronize());
   YH_HIP(hipMemset(pl->apd.front, 0, pl->n * sizeof(float)));
   YH_HIP(hipMemset(pl->apd.back, 0, pl->n * sizeof(float)));

@@ -742,13 +742,14 @@ int yh_sr_begin(yh_plan* pl, float tip0x, float tip0y, void* stream) {
     const int box = 2 * sr->reach + 1;
     const int gx = (box + kSliceBX - 1) / kSliceBX, gy = (box + kSliceBY - 1) / kSliceBY;
     sr->nparts = gx * gy;
-    YH_HIP(hipMalloc(&sr->st, 2 * sizeof(SrDev)));
-    YH_HIP(hipMalloc(&sr->partials, sizeof(double) * 12 * sr->nparts));
-    YH_HIP(hipMalloc(&sr->step_tips, sizeof(yh_tip) * kSrTipCap));
-    YH_HIP(hipMalloc(&sr->step_keys, sizeof(int) * kSrTipCap));
-    YH_HIP(hipMalloc(&sr->step_count, sizeof(int)));
   }
   yh_sr* sr = pl->sr;
+  // each buffer on its own, so a call after a failed allocation completes it
+  if (!sr->st) YH_HIP(hipMalloc(&sr->st, 2 * sizeof(SrDev)));
+  if (!sr->partials) YH_HIP(hipMalloc(&sr->partials, sizeof(double) * 12 * sr->nparts));
+  if (!sr->step_tips) YH_HIP(hipMalloc(&sr->step_tips, sizeof(yh_tip) * kSrTipCap));
+  if (!sr->step_keys) YH_HIP(hipMalloc(&sr->step_keys, sizeof(int) * kSrTipCap));
+  if (!sr->step_count) YH_HIP(hipMalloc(&sr->step_count, sizeof(int)));
   SrDev h[2];
   std::memset(h, 0, sizeof h);
   h[0].tip0x = h[1].tip0x = tip0x;
