@@ -62,7 +62,8 @@ def parse():
                     default=[0, 4])
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-traffic", action="store_true",
-                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
+                    help="skip the rocprofv3 counter passes behind roofline.traffic and "
+                         "roofline.valu_issue")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant-kernel timing (for profiling); needs --cfg")
     return ap.parse_args()
@@ -177,6 +178,31 @@ def hbm_traffic(a, cfg):
                           "scale": round(factor, 4)}
         total += kib * factor * 1024.0
     return total, parts
+
+
+VALU_CLOCK_GHZ = 2.4  # MI355X peak engine clock
+
+
+def valu_issue(a, cfg, t_launch_ms):
+    """VALU wave instructions per launch of the dominant kernel (SQ_INSTS_VALU,
+    one rocprofv3 pass) and the time the chip's SIMDs need to issue them at
+    the peak engine clock: a wave instruction occupies a 16-lane SIMD for 4
+    cycles, 4 SIMDs per CU.  Transcendentals take longer, so `frac` (issue
+    time / launch time) is a lower bound on how busy the VALUs are.  None if
+    the pass fails."""
+    import torch
+    rows = _counter_pass(a, cfg, "SQ_INSTS_VALU")
+    if not rows:
+        return None
+    kern = [v for k, vs in rows.items() if "k_tiled" in k for v in vs]
+    if not kern:
+        return None
+    insts = float(np.mean(kern[len(kern) // 5:]))
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    issue_us = insts * 4.0 / (4 * cus) / (VALU_CLOCK_GHZ * 1e3)
+    return {"counter": "SQ_INSTS_VALU", "insts_per_launch": round(insts), "cus": cus,
+            "clock_GHz": VALU_CLOCK_GHZ, "issue_us": round(issue_us, 3),
+            "frac": round(issue_us / (t_launch_ms * 1e3), 4)}
 
 
 def kernel_timing(a, pl, cfg, stream, n, bytes_per_cell, launches=None):
@@ -484,7 +510,7 @@ def main():
         f"({n * K / (t_launch_ms / 1e3) / 1e9:.1f} Gcell-steps/s inside one launch)")
 
     pl.close()
-    traffic = None
+    traffic = valu = None
     if not a.no_traffic and cfg[0] == T.YH_KERNEL_TILED:
         log("HBM traffic: FETCH_SIZE and WRITE_SIZE passes under rocprofv3 ...")
         traffic = hbm_traffic(a, cfg)
@@ -493,6 +519,11 @@ def main():
                 f"({traffic[0] / alg_bytes:.3f}x the algorithmic {alg_bytes / 1e6:.2f} MB)")
         else:
             log("HBM traffic: counter passes unavailable")
+        valu = valu_issue(a, cfg, t_launch_ms)
+        if valu:
+            log(f"VALU: {valu['insts_per_launch'] / 1e6:.3f} M wave instructions per launch = "
+                f"{valu['issue_us']:.2f} us of issue at {VALU_CLOCK_GHZ} GHz "
+                f"({valu['frac']:.3f} of the launch)")
     copy_gbps = copy_bandwidth(stream)
     log(f"device copy of 256 MiB: {copy_gbps:.0f} GB/s ({copy_gbps / HBM_PEAK_GBPS:.3f} of peak)")
     control = None
@@ -553,6 +584,7 @@ def main():
             "steps_per_launch": K,
             "equiv_GBps_per_step": round(bytes_per_cell * n * K / (t_launch_ms / 1e3) / 1e9, 1),
             "copy_GBps": round(copy_gbps, 1),
+            "valu_issue": valu,
         },
         "cpu_baseline": cpu,
         "batched": batch,

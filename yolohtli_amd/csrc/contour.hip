@@ -84,6 +84,10 @@ __global__ void k_sapd(const float* __restrict__ apd1, const float* __restrict__
     sapd[q] = (apd1[q] - apd2[q] > 0.0f) ? 1.0f : -1.0f;
 }
 
+// Zeroes the hit count on the stream with a kernel, not a memset, so the
+// entry point can be captured into a graph.
+__global__ void k_zero_count(int* count) { *count = 0; }
+
 static hipError_t launch_contour(int nx, int ny, int mode, double th1, double th2, double th3,
                                  const float* f1, const float* f2, const uint8_t* area, float t,
                                  float* pts, int* keys, int* count, int cap, hipStream_t s) {
@@ -109,7 +113,8 @@ int yh_contour_dev(int nx, int ny, int mode, double th1, double th2, double th3,
   if ((int64_t)nx * ny > (int64_t)1 << 30) return YH_EINVAL;  // keys 2*cell + sub in int32
   hipStream_t s = (hipStream_t)stream;
   // countour_wrapper zeroes the count before every search (spaceAPD.cu:261)
-  YH_HIP(hipMemsetAsync(d_count, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, d_count);
+  YH_HIP(hipGetLastError());
   YH_HIP(launch_contour(nx, ny, mode, th1, th2, th3, field1, field2, area, time, d_pts, d_keys,
                         d_count, cap, s));
   return YH_OK;

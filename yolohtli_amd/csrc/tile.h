@@ -261,7 +261,7 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
               if (!APD) {
 #pragma unroll
                 for (int a = 0; a < NV; ++a)
-                  __builtin_nontemporal_store(h ? 0.0f : X[a][r][c], at32(out.f[a], idx));
+                  *at32(out.f[a], idx) = h ? 0.0f : X[a][r][c];
               }
             }
           }
