@@ -490,7 +490,7 @@ int yh_batch_autotune(yh_batch* b, int steps, void* stream, yh_kernel_cfg* best,
   for (int t = 0; t < b->nt; ++t) minside = std::min(minside, std::min(b->p[t].nx, b->p[t].ny));
   const int shapes[][3] = {{4, 8, 1}, {8, 8, 1}, {4, 16, 1}, {8, 16, 1}, {7, 16, 1},
                            {4, 8, 2}, {4, 16, 2}};
-  const int Ks[] = {2, 4, 6, 8, 12};
+  const int Ks[] = {2, 4, 6, 8, 9, 10, 12};
   yh_kernel_cfg bestc = cfg0;
   int bestp = pers0;
   float bestt = 1e30f;
