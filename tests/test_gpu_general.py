@@ -160,6 +160,44 @@ def test_contours_match_oracle(torch_cuda, mode, th):
     np.testing.assert_allclose(got, want, rtol=0, atol=2e-5)
 
 
+def test_contour_dev_graph_replays_match_eager(torch_cuda):
+    """yh_contour_dev captured into a graph: every replay zeroes the count on
+    the stream (with a kernel, not a memset node) and finds the same points as
+    the eager call, with eager work and synchronisations between replays."""
+    torch = torch_cuda
+    nx, ny, cap, th = 96, 80, 2 * 96 * 80, (0.5, 0.25, 0.0)
+    j, i = np.mgrid[0:ny, 0:nx]
+    f1 = (0.5 + 0.5 * np.sin(0.11 * i + 0.7) * np.cos(0.07 * j)).astype(np.float32)
+    f2 = (np.sin(0.09 * i - 0.3) * np.cos(0.13 * j + 0.2) * 0.6 + 0.3).astype(np.float32)
+    d1, d2 = torch.from_numpy(f1.reshape(-1)).cuda(), torch.from_numpy(f2.reshape(-1)).cuda()
+    pts = torch.zeros((cap, 3), dtype=torch.float32, device="cuda")
+    keys = torch.zeros(cap, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+    def result():
+        torch.cuda.synchronize()
+        n = int(cnt.item())
+        k = keys[:n].cpu().numpy()
+        order = np.argsort(k)
+        return k[order], pts[:n].cpu().numpy()[order]
+
+    Y.solver.countour_wrapper(nx, ny, d1, d2, None, pts, keys, cnt, cap, 3.5, 2, th)
+    wk, wp = result()
+    assert len(wk) > 20
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        Y.solver.countour_wrapper(nx, ny, d1, d2, None, pts, keys, cnt, cap, 3.5, 2, th)
+    for rep in range(3):
+        cnt.fill_(777 + rep)
+        keys.fill_(-1)
+        pts.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        gk, gp = result()
+        np.testing.assert_array_equal(gk, wk)
+        np.testing.assert_array_equal(gp, wp)
+
+
 def test_plan_contour_modes(torch_cuda):
     """Plan contours of (u, v) and, with APD on, of sign(APD1 - APD2)."""
     p = T.grid_params(64, 64)
