@@ -165,10 +165,12 @@ typedef struct yh_plan yh_plan;
 enum { YH_KERNEL_AUTO = 0, YH_KERNEL_DIRECT = 1, YH_KERNEL_TILED = 2 };
 typedef struct yh_kernel_cfg {
   int variant;   /* YH_KERNEL_* */
-  int substeps;  /* k fused time steps per launch (TILED), 1..16 */
-  int rows;      /* rows held in registers per thread (TILED): 4, 8 or 16 */
+  int substeps;  /* k fused time steps per launch (TILED), 1..16, 2k < rows*waves */
+  int rows;      /* rows held in registers per thread (TILED): 4, 7, 8 or 16 */
   int waves;     /* wavefronts per workgroup (TILED): 4, 8 or 16 */
-  int cols;      /* columns per lane (TILED): 1 or 2 */
+  int cols;      /* columns per lane (TILED): 1 or 2.  Built shapes (rows x
+                  * waves x cols): 4x4, 8x4, 16x4, 4x8, 8x8, 16x8, 4x16, 8x16,
+                  * 7x16 with cols 1; 4x4, 8x4, 4x8, 8x8, 4x16 with cols 2 */
 } yh_kernel_cfg;
 
 /* Creates a plan: checks parameters, allocates ping-pong state on the current

@@ -157,7 +157,7 @@ bool tiled_supported(int rows, int waves, int cols) {
   const bool rw = (rows == 4 && (waves == 4 || waves == 8 || waves == 16)) ||
                   (rows == 8 && (waves == 4 || waves == 8 || waves == 16)) ||
                   (rows == 16 && (waves == 4 || waves == 8)) ||
-                  (rows == 7 && waves == 16 && cols == 1);  // 64 x 112: 242 tiles at 1024^2
+                  (rows == 7 && waves == 16 && cols == 1);  // 64 x 112: 253 tiles at 1024^2, K=9
   // two columns per lane double the registers: 16 rows, or 8 rows in 16 waves, spill
   return rw && (cols == 1 || (cols == 2 && rows * waves <= 64 && rows <= 8));
 }
