@@ -422,8 +422,8 @@ def adjoint(a):
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the multi-GPU (domain-decomposed) path is a design "
-              "only in this build; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the multi-GPU (domain-decomposed) stepper is not "
+              "implemented; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
     import yolohtli_amd as Y
