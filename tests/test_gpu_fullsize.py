@@ -1,0 +1,107 @@
+"""Full-size runs of the plain stepper on the GPU, checked through properties
+that do not depend on an oracle run of the same length (the FP64 oracle
+would take minutes for 50000 steps at 1024^2; the first 200 steps of the
+headline workload are compared with it in test_gpu_stepper.py):
+
+  * the 1024^2 masked FK workload over 50000 steps with tips every 100
+    (main.cu:876-885, 961-964): the state stays finite, hole cells stay 0
+    (reactionDiffusion.cu:521-522), and the FK gates stay in [0, 1] exactly
+    (each gate update is fma(b, t - g, g) with a rate b in [0, 1] and a
+    target t of 0 or 1, which cannot round past t);
+  * from the cross-field start (initGates, main.cu:577-621), which forms a
+    spiral, the same run logs tips, all inside the centred disk the masked
+    search covers (tipTracker.cu:47-51) and in time order;
+  * a state that is mirror symmetric in x and in y stays so bit for bit: the
+    unmasked 5-point sum adds (W + E) + (N + S), both commutative, and the
+    Neumann ghost cells mirror the domain (helper_functions.cu:69-79), so an
+    index, reflection or halo error of the tiled kernels breaks the symmetry.
+    The start is the 512^2 cross-field state mirrored into the four quadrants:
+    a mirror-symmetric solution evolves like that 512^2 run with no-flux
+    walls on the symmetry lines, so its spirals stay active.
+"""
+import numpy as np
+import pytest
+
+import yolohtli_amd as Y
+from yolohtli_amd import abi_types as T
+
+pytestmark = pytest.mark.gpu
+
+NX = NY = 1024
+STEPS = 50000
+
+
+def _masked_run(fields):
+    p = T.grid_params(NX, NY, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    mask = Y.make_holes(NX, NY, 200, 3.0, 12.0, Y.seed(1, 0, 1))
+    pl = Y.Plan(p)
+    pl.set_mask(mask)
+    pl.set_state(*fields)
+    pl.set_kernel(T.YH_KERNEL_TILED, 9, 7, 16, 1)
+    for _ in range(STEPS // 100):
+        pl.step(100)
+        pl.track_tips()
+    assert pl.check_finite() == 0
+    state = pl.get_state()
+    tips = pl.read_tips()
+    pl.close()
+    return p, mask, state, tips
+
+
+def _check_tips(p, tips):
+    if len(tips) == 0:
+        return
+    r = np.sqrt(p.tipOffsetX * p.tipOffsetY)
+    d2 = (tips[:, 0] - NX // 2) ** 2 + (tips[:, 1] - NY // 2) ** 2
+    assert np.all(d2 < (r + 1.5) ** 2)
+    t = tips[:, 4]
+    assert np.all(np.diff(t) >= 0)  # the log is ordered by time
+    assert t.min() > 0 and t.max() <= STEPS * 0.02 * (1 + 1e-6)
+
+
+def test_headline_workload_invariants_over_50000_steps(torch_cuda):
+    f = Y.init_random(NX, NY, Y.seed(1, 0, 2), gate=1.0, nvars=3)
+    p, mask, (u, v, w), tips = _masked_run(f)
+    hole = mask == 0
+    for a in (u, v, w):
+        assert np.all(a[hole] == 0.0)
+    for g in (v, w):
+        assert g.min() >= 0.0 and g.max() <= 1.0, (g.min(), g.max())
+    assert -0.5 < u.min() and u.max() < 2.0, (u.min(), u.max())
+    _check_tips(p, tips)
+
+
+def test_cross_field_start_logs_tips_in_the_disk(torch_cuda):
+    p, mask, (u, v, w), tips = _masked_run(Y.initGates(NX, NY, nvars=3))
+    assert np.all(u[mask == 0] == 0.0)
+    for g in (v, w):
+        assert g.min() >= 0.0 and g.max() <= 1.0
+    assert len(tips) > 0
+    _check_tips(p, tips)
+
+
+@pytest.mark.parametrize("model,cfg", [
+    (T.YH_MODEL_3V_FK, (T.YH_KERNEL_TILED, 9, 7, 16, 1)),
+    (T.YH_MODEL_3V_FK, (T.YH_KERNEL_TILED, 8, 8, 8, 1)),
+    (T.YH_MODEL_2V_REF, (T.YH_KERNEL_TILED, 8, 4, 16, 2)),
+])
+def test_mirror_symmetry_is_kept_bit_for_bit(torch_cuda, model, cfg):
+    p = T.grid_params(NX, NY, model=model)
+    nv = T.nvars(p)
+    quarter = Y.initGates(NX // 2, NY // 2, nvars=nv)
+    sym = []
+    for a in quarter:
+        q = a.reshape(NY // 2, NX // 2)
+        top = np.concatenate([q, q[:, ::-1]], axis=1)
+        sym.append(np.ascontiguousarray(np.concatenate([top, top[::-1, :]], axis=0)).reshape(-1))
+    pl = Y.Plan(p)
+    pl.set_state(*sym)
+    pl.set_kernel(*cfg)
+    pl.step(20000)
+    out = [a.reshape(NY, NX) for a in pl.get_state()]
+    pl.close()
+    for a in out:
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, a[:, ::-1])
+        np.testing.assert_array_equal(a, a[::-1, :])
+    assert out[0].std() > 1e-3  # spirals still active, not a uniform state
