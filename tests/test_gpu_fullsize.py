@@ -17,7 +17,9 @@ headline workload are compared with it in test_gpu_stepper.py):
     index, reflection or halo error of the tiled kernels breaks the symmetry.
     The start is the 512^2 cross-field state mirrored into the four quadrants:
     a mirror-symmetric solution evolves like that 512^2 run with no-flux
-    walls on the symmetry lines, so its spirals stay active.
+    walls on the symmetry lines, so its spirals stay active;
+  * the full 256-tissue batched workload stays finite with holes 0 and gates
+    in [0, 1], and tissues of every size equal single-tissue plan runs.
 """
 import numpy as np
 import pytest
@@ -105,3 +107,41 @@ def test_mirror_symmetry_is_kept_bit_for_bit(torch_cuda, model, cfg):
         np.testing.assert_array_equal(a, a[:, ::-1])
         np.testing.assert_array_equal(a, a[::-1, :])
     assert out[0].std() > 1e-3  # spirals still active, not a uniform state
+
+
+def test_full_batched_workload_matches_single_tissue_plans(torch_cuda):
+    """The 256-tissue batched workload (50M cells, sides 128..768, parameters
+    +-20%, 0..40 holes) over 1000 steps in one captured graph of work-list
+    launches, K=6 on 8 x 8 waves: every tissue finite, holes 0, gates in
+    [0, 1]; four tissues (the smallest, the largest and two others) equal bit
+    for bit to single-tissue plans taking the same 6-step launches."""
+    from yolohtli_amd.batch import Batch, batched_workload
+    params, masks, fields = batched_workload(ntissues=256)
+    b = Batch(params)
+    for t in range(len(params)):
+        b.set_mask(t, masks[t])
+        b.set_state(t, *fields[t])
+    b.set_kernel(substeps=6, rows=8, waves=8, cols=1, persistent=0)
+    b.set_graph(True)
+    b.step(1000)
+    sides = [p.nx for p in params]
+    pick = {int(np.argmin(sides)), int(np.argmax(sides)), 7, 100}
+    for t in range(len(params)):
+        u, v, w = b.get_state(t)
+        hole = masks[t] == 0
+        assert np.isfinite(u).all()
+        for a in (u, v, w):
+            assert np.all(a[hole] == 0.0)
+        for g in (v, w):
+            assert g.min() >= 0.0 and g.max() <= 1.0
+        if t in pick:
+            pl = Y.Plan(params[t])
+            pl.set_mask(masks[t])
+            pl.set_state(*fields[t])
+            pl.set_option(T.YH_OPT_KEEP_PREV, 0)
+            pl.set_kernel(T.YH_KERNEL_TILED, 6, 8, 8, 1)
+            pl.step(1000)
+            for x, y in zip((u, v, w), pl.get_state()):
+                np.testing.assert_array_equal(x, y)
+            pl.close()
+    b.close()
