@@ -19,7 +19,9 @@ headline workload are compared with it in test_gpu_stepper.py):
     a mirror-symmetric solution evolves like that 512^2 run with no-flux
     walls on the symmetry lines, so its spirals stay active;
   * the full 256-tissue batched workload stays finite with holes 0 and gates
-    in [0, 1], and tissues of every size equal single-tissue plan runs.
+    in [0, 1], and tissues of every size equal single-tissue plan runs;
+  * the 512^2 symmetry-reduction workload matches the oracle over its first
+    40 steps and keeps the reference's phi update over 20000 more.
 """
 import numpy as np
 import pytest
@@ -145,3 +147,66 @@ def test_full_batched_workload_matches_single_tissue_plans(torch_cuda):
                 np.testing.assert_array_equal(x, y)
             pl.close()
     b.close()
+
+
+def test_sr_at_full_size(torch_cuda):
+    """The symmetry-reduction workload at its full size (512^2 FK, the disk of
+    radius 160, dt = 0.01 ms), from the cross-field start warmed up by plain
+    steps until a tip exists and 2000 steps more (a formed spiral with one
+    tip): the first 40 co-moving-frame steps against the FP64 oracle (c and
+    phi history 1e-3 relative, state 1e-4 absolute, as at 128^2 in
+    test_gpu_sr.py), then 20000 more steps in which every step's recorded phi
+    follows the Euler update of main.cu:935-941 from the previous one
+    (phi_k+1 = phi_k + dt * c_k+1, to FP64 rounding), no solve is singular and
+    the state stays finite.  (Right after the first tip, the wave next to the
+    wall can give near-degenerate extra tips that an FP64 and an FP32 state
+    resolve differently; the disk then follows the last tip of each and the
+    runs part, as the reference's choice of tip_vector[count-1] implies.)"""
+    import oracle.oracle as O
+    n = 512
+    p0 = T.grid_params(n, n, model=T.YH_MODEL_3V_FK)
+    pl = Y.Plan(p0)
+    pl.set_state(*Y.initGates(n, n, u_cols=32, rest=1.0, refr=0.0, nvars=3))
+    tips = []
+    for _ in range(200):
+        pl.step(100)
+        pl.track_tips()
+        tips = pl.read_tips()
+        if len(tips):
+            break
+    assert len(tips) > 0
+    pl.clear_tips()
+    pl.step(2000)
+    pl.track_tips()
+    tips = pl.read_tips()
+    st = pl.get_state()
+    pl.close()
+    assert len(tips) == 1
+    tip0 = (float(tips[-1, 0]), float(tips[-1, 1]))
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, reduceSym=1)
+    pl = Y.Plan(p)
+    pl.set_state(*st)
+    pl.sr_begin(tip0)
+    hist = pl.sr_step(40, history=True)
+    got = pl.get_state()
+    ref = O.SR(p, [x.astype(np.float64) for x in st], tip0=tip0)
+    want = []
+    for _ in range(40):
+        want.append(np.concatenate([ref.c, ref.phi]))
+        ref.step(1)
+    want = np.array(want)
+    scale = np.max(np.abs(want), axis=0) + 1e-12
+    assert np.all(np.max(np.abs(hist - want) / scale, axis=0) < 1e-3)
+    for a in range(3):
+        assert np.max(np.abs(got[a] - ref.X[a])) < 1e-4
+    more = pl.sr_step(20000, history=True)
+    info = pl.sr_info()
+    assert pl.check_finite() == 0
+    pl.close()
+    h = np.concatenate([hist, more])
+    assert np.isfinite(h).all()
+    assert info["status"] & 1 == 0 and info["count"] == 20040
+    dt = O.derive(p).dt
+    phi_next = h[:-1, 3:] + dt * h[1:, :3]
+    tol = 4 * np.finfo(np.float64).eps * (np.abs(h[:-1, 3:]) + np.abs(dt * h[1:, :3]))
+    assert np.all(np.abs(h[1:, 3:] - phi_next) <= tol)
