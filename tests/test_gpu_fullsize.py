@@ -21,7 +21,9 @@ headline workload are compared with it in test_gpu_stepper.py):
   * the full 256-tissue batched workload stays finite with holes 0 and gates
     in [0, 1], and tissues of every size equal single-tissue plan runs;
   * the 512^2 symmetry-reduction workload matches the oracle over its first
-    40 steps and keeps the reference's phi update over 20000 more.
+    40 steps and keeps the reference's phi update over 20000 more;
+  * the 256^2 x 2000-step gradient workload matches the FP64 oracle adjoint
+    and central differences of the FP64 loss.
 """
 import numpy as np
 import pytest
@@ -210,3 +212,49 @@ def test_sr_at_full_size(torch_cuda):
     phi_next = h[:-1, 3:] + dt * h[1:, :3]
     tol = 4 * np.finfo(np.float64).eps * (np.abs(h[:-1, 3:]) + np.abs(dt * h[1:, :3]))
     assert np.all(np.abs(h[1:, 3:] - phi_next) <= tol)
+
+
+def test_adjoint_at_full_size(torch_cuda):
+    """The gradient workload at its full size (256^2 FK with a tanh-smoothed
+    Heaviside, 2000 steps, 64 sensors recorded every step, checkpoints every
+    100, targets from the hidden D field, gradient at the uniform D0): the
+    loss, the D-gradient field and the tau-gradient against the FP64 oracle
+    adjoint (1e-3 relative), and two directional checks against FP64 central
+    differences of the oracle's loss: tau_d, and D along a smooth bump in the
+    band the wave crosses."""
+    torch = torch_cuda
+    import oracle.oracle as O
+    from yolohtli_amd.adjoint import SensorProblem, TAU_NAMES, adjoint_workload
+    n, nsteps = 256, 2000
+    p, x0, sensors, Dh, D0 = adjoint_workload(n=n, nsteps=nsteps)
+    pr = SensorProblem(p, nsteps, sensors, x0, rec_every=1, ckpt_every=100)
+    _, targets = pr.forward(torch.from_numpy(Dh.reshape(-1)).cuda())
+    pr.set_targets(targets)
+    L, _ = pr.forward(torch.full((n * n,), D0, dtype=torch.float32, device="cuda"))
+    gD, gt = pr.backward()
+    pr.close()
+    gD = gD.cpu().numpy().astype(np.float64)
+    X0 = [f.astype(np.float64) for f in x0]
+    Dd = np.full(n * n, D0, np.float64)
+    tg = np.asarray(targets, np.float64)
+    Lo, gDo, gto = O.fk_adjoint(p, Dd, X0, nsteps, 1, sensors, tg, None)
+    assert L == pytest.approx(Lo, rel=1e-3)
+    assert np.linalg.norm(gD - gDo) / np.linalg.norm(gDo) < 1e-3
+    for k in range(6):
+        assert gt[k] == pytest.approx(gto[k], rel=1e-3), (TAU_NAMES[k], gt[k], gto[k])
+
+    def loss(pp, D):
+        return O.fk_loss(pp, D, X0, nsteps, 1, sensors, tg)
+
+    k = TAU_NAMES.index("tau_d")
+    t0 = p.fk.tau_d
+    h = 1e-4 * t0
+    fd = (loss(T.copy_params(p, fk={"tau_d": t0 + h}), Dd)
+          - loss(T.copy_params(p, fk={"tau_d": t0 - h}), Dd)) / (2 * h)
+    assert gt[k] == pytest.approx(fd, rel=1e-3)
+    jj, ii = np.mgrid[0:n, 0:n].astype(np.float64)
+    bump = np.exp(-((ii - 0.15 * n) ** 2 + (jj - 0.5 * n) ** 2) / (2 * (0.05 * n) ** 2))
+    bump = bump.reshape(-1)
+    h = 1e-3 * D0
+    fd = (loss(p, Dd + h * bump) - loss(p, Dd - h * bump)) / (2 * h)
+    assert float(gD @ bump) == pytest.approx(fd, rel=1e-3)
