@@ -202,12 +202,9 @@ def test_tips_after_multistep_call_match_oracle(torch_cuda, cfg, masked):
     log = a.read_tips()
     fa = a.get_state()
     # 36 steps in K-substep launches, then the last step as the first plan
-    # takes it: inside a tiled launch (fused) or as the single-step tail.
-    # Unmasked, reflected halo cells mirror the domain bit for bit, so the two
-    # splits agree.  With a mask the stencil adds the W/E and N/S terms of a
-    # halo cell in mirrored order, which can change the last bit of cells at
-    # the domain edge; for this mask and state the splits still agree bit for
-    # bit (checked below).
+    # takes it: inside a tiled launch (fused) or as the single-step tail
+    # (a tiled cell update does not depend on where launches split: reflected
+    # halo cells stay bitwise mirrors of the cells they reflect)
     b.set_option(T.YH_OPT_KEEP_PREV, 0)
     b.step(36)
     s36 = b.get_state()
@@ -223,6 +220,65 @@ def test_tips_after_multistep_call_match_oracle(torch_cuda, cfg, masked):
         np.testing.assert_array_equal(x, y)
     assert len(log) == len(want)
     np.testing.assert_allclose(log, want, rtol=0, atol=1e-5)
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_tiled_results_do_not_depend_on_launch_split(torch_cuda, model):
+    """With one column per lane, reflected halo cells add their stencil terms
+    in the order of the cells they mirror, so they stay bitwise mirrors and a
+    cell's value after n steps does not depend on the fused depth, the tile
+    shape or where calls split: masked runs of 100 steps with K = 3, 5, 8, 9
+    on four shapes, and in calls of 37 + 63, agree bit for bit.  The previous
+    state is not kept, so every step runs in the tiled kernel (a shape built
+    without the fused previous-u output would end each call with the
+    one-step kernel, whose stencil sums in another order)."""
+    n = 256
+    p = T.grid_params(n, n, model=model, solidSwitch=1)
+    mask = Y.make_holes(n, n, 12, 3.0, 9.0, Y.seed(5, 0, 1))
+    f = Y.init_random(n, n, Y.seed(5, 0, 2), gate=1.0, nvars=T.nvars(p))
+    runs = []
+    for cfg, calls in (((3, 8, 4, 1), [100]), ((5, 4, 16, 1), [100]), ((8, 8, 8, 1), [100]),
+                       ((9, 7, 16, 1), [100]), ((9, 7, 16, 1), [37, 63])):
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        pl.set_kernel(T.YH_KERNEL_TILED, *cfg)
+        pl.set_option(T.YH_OPT_KEEP_PREV, 0)
+        for c in calls:
+            pl.step(c)
+        runs.append(pl.get_state())
+        pl.close()
+    for r in runs[1:]:
+        for x, y in zip(runs[0], r):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_remainder_merge_is_bitwise_equal(torch_cuda):
+    """A 100-step call with K = 9 on 7 rows x 16 waves ends with one launch of
+    10 substeps on 8 rows x 16 waves (the remainder merged into the last full
+    launch: at 512^2 that shape needs 60 tiles against 72).  State and tips
+    equal those of eleven 9-step calls and one 1-step call, which merge
+    nothing."""
+    n = 512
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    mask = Y.make_holes(n, n, 50, 3.0, 12.0, Y.seed(1, 0, 1))
+    f = Y.initGates(n, n, nvars=3)
+    out = []
+    for calls in ([100], [9] * 11 + [1]):
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        pl.set_kernel(T.YH_KERNEL_TILED, 9, 7, 16, 1)
+        for rep in range(30):
+            for c in calls:
+                pl.step(c)
+            pl.track_tips()
+        out.append((pl.get_state(), pl.read_tips()))
+        pl.close()
+    for x, y in zip(out[0][0], out[1][0]):
+        np.testing.assert_array_equal(x, y)
+    assert len(out[0][1]) > 0
+    np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
 def test_check_finite_and_argument_errors(torch_cuda):

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -30,6 +31,32 @@ yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
     }
   }
   return c;
+}
+
+// Tiles of one launch of K substeps on a rows x waves x cols shape.
+static int64_t tile_count(int nx, int ny, int rows, int waves, int cols, int K) {
+  const int wout = 64 * cols - 2 * K, hout = rows * waves - 2 * K;
+  if (wout <= 0 || hout <= 0) return INT64_MAX;
+  return (int64_t)((nx + wout - 1) / wout) * ((ny + hout - 1) / hout);
+}
+
+// Whether a remainder of r < K steps can be merged into the last K-step
+// launch of a call: the shape one row taller per wave (same waves, one
+// column per lane, where a tiled update does not depend on where launches
+// split) is built, with the previous-u output when `prev`, and runs K + r
+// substeps on no more tiles than a K-step launch of the plan's shape, so
+// the merged launch occupies no more CUs and the call takes one launch
+// fewer (a launch's fixed part is about a third of a K=9 launch at 1024^2).
+static bool merge_ok(const yh_plan* pl, const yh_kernel_cfg& c, int r, bool prev) {
+  const int K = c.substeps, rows = c.rows + 1, Km = K + r;
+  if (c.cols != 1 || !tiled_supported(rows, c.waves, c.cols)) return false;
+  if (prev && !tiled_prev_supported(rows, c.waves, c.cols)) return false;
+  const int nx = pl->p.nx, ny = pl->p.ny;
+  // the reflected halo is exact for at most min(nx, ny) - 1 substeps
+  if (Km > 16 || Km > std::min(nx, ny) - 1) return false;
+  if (2 * Km >= 64 * c.cols || 2 * Km >= rows * c.waves) return false;
+  return tile_count(nx, ny, rows, c.waves, c.cols, Km) <=
+         tile_count(nx, ny, c.rows, c.waves, c.cols, K);
 }
 
 int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
@@ -77,14 +104,20 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
   const int tail = (pl->keep_prev && !prev_fused) ? 1 : 0;
   if (prev_fused && !pl->prev_u) YH_HIP(hipMalloc(&pl->prev_u, pl->n * sizeof(float)));
   if (tiled) {
+    const int body = remaining - tail, r = body % c.substeps;
+    const bool merge = !ap && r > 0 && body > c.substeps && merge_ok(pl, c, r, prev_fused);
     while (remaining > tail) {
-      const int K = std::min(c.substeps, remaining - tail);
+      int K = std::min(c.substeps, remaining - tail), rows = c.rows;
+      if (merge && remaining - tail == c.substeps + r) {  // the last full launch + the rest
+        K = c.substeps + r;
+        rows = c.rows + 1;
+      }
       CFields ci;
       Fields co;
       fields(pl->cur, &ci, &co);
       apd.count0 = (int)done;
       float* prev = (prev_fused && K == remaining) ? pl->prev_u : nullptr;
-      YH_HIP(launch_tiled(pl->p.model, masked, c.rows, c.waves, c.cols, K, pl->k, ci, co,
+      YH_HIP(launch_tiled(pl->p.model, masked, rows, c.waves, c.cols, K, pl->k, ci, co,
                           masked ? pl->codes_d : nullptr, s, ap, prev));
       pl->cur ^= 1;
       remaining -= K;

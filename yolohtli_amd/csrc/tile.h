@@ -112,7 +112,9 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
         const bool h = b == kHole;
         const uint32_t q = h ? 0u : b;
         uint32_t w2 = q & 3u, e2 = (q >> 2) & 3u, n2 = (q >> 4) & 3u, s2 = (q >> 6) & 3u;
-        if (flip_x[c]) { const uint32_t t = w2; w2 = e2; e2 = t; }
+        // a reflected column keeps the weights of the cell it mirrors when
+        // its lane swaps its neighbour sources below (one column per lane)
+        if (CW != 1 && flip_x[c]) { const uint32_t t = w2; w2 = e2; e2 = t; }
         if (flip_y) { const uint32_t t = n2; n2 = s2; s2 = t; }
         const int rm = MASKED ? r : 0;
         cw[rm][c] = (float)w2;
@@ -149,8 +151,20 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
     }
   }
 
-  const int addr_w = ((lane + 63) & 63) << 2;
-  const int addr_e = ((lane + 1) & 63) << 2;
+  // ds_bpermute sources of the west and east neighbours.  A reflected halo
+  // column (one column per lane) takes them the other way round and keeps
+  // its mirror cell's weights, so it adds the same terms in the same order
+  // as that cell and stays its bitwise mirror through the substeps: results
+  // then do not depend on where launches split, with or without a mask.
+  // (The N/S pair needs nothing: its first product is exact for integer
+  // weights, so the fma that adds the second rounds the exact pair sum.)
+  int addr_w = ((lane + 63) & 63) << 2;
+  int addr_e = ((lane + 1) & 63) << 2;
+  if (CW == 1 && flip_x[0]) {
+    const int t = addr_w;
+    addr_w = addr_e;
+    addr_e = t;
+  }
   [[maybe_unused]] const typename M::Pinned pinned = M::pin(k);
   const float dscale = k.iso ? k.trx : 1.0f;  // the laplacian terms below carry trx otherwise
 
@@ -236,10 +250,7 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
           if constexpr (MODEL == MFK) {
             M::step(k, pinned, dif[0][c], X[0][r][c], X[1][r][c], X[NV - 1][r][c]);
           } else {
-            float du, dv, dw;
-            M::react(k, X[0][r][c], X[1][r][c], 0.0f, du, dv, dw);
-            X[0][r][c] = dif[0][c] + du;
-            X[1][r][c] += __builtin_fmaf(k.rs, dif[ND - 1][c], dv);
+            M::step(k, dif[0][c], dif[ND - 1][c], X[0][r][c], X[1][r][c]);
           }
           if (APD && !(MASKED && ((hole[c] >> r) & 1u))) {
             apd_cell(u_old, X[0][r][c], apd.count0 + s + 1, apd.dt, apd.th,

@@ -107,6 +107,21 @@ struct Model<M2V> {
     const float t3 = (u - k.gamma) * (k.beta - u);
     dv = __builtin_fmaf(-k.a2_F, v, __builtin_fmaf(k.a2_E, t3, -k.a2_G));
   }
+  // The Euler update of one cell in the tiled kernels: ud = u plus its
+  // diffusion term, vd = v's diffusion term.  u' = ud + u (s3 - B v) is one
+  // written-out fma, so no product is left for the compiler to contract with
+  // the caller's code: every inlined copy (the first substep is peeled, and
+  // each tile shape is its own instantiation) rounds a cell identically.
+  __device__ __forceinline__ static void step(const KParams& k, float ud, float vd, float& u,
+                                              float& v) {
+    const float s2 = __builtin_fmaf(u, -k.a2_A, k.a2_A);
+    const float s3 = (u - k.alpha) * s2;
+    const float t3 = (u - k.gamma) * (k.beta - u);
+    const float dv = __builtin_fmaf(-k.a2_F, v, __builtin_fmaf(k.a2_E, t3, -k.a2_G));
+    const float un = __builtin_fmaf(u, __builtin_fmaf(-k.a2_B, v, s3), ud);
+    v = v + __builtin_fmaf(k.rs, vd, dv);
+    u = un;
+  }
   struct Pinned {};
   __device__ __forceinline__ static Pinned pin(const KParams&) { return {}; }
 };
