@@ -189,7 +189,11 @@ int yh_set_kernel(yh_plan* plan, const yh_kernel_cfg* cfg);
 int yh_get_kernel(const yh_plan* plan, yh_kernel_cfg* cfg);
 /* Advance nsteps explicit-Euler steps (the display() plain loop,
  * main.cu:876-885).  After return the plan also holds the state one step
- * earlier, so yh_track_tips sees (u^{n-1}, u^n) as main.cu:961-964 does. */
+ * earlier, so yh_track_tips sees (u^{n-1}, u^n) as main.cu:961-964 does.
+ * With a tiled kernel of one column per lane, a remainder of nsteps % K
+ * steps may run inside the call's last launch on the shape one row taller
+ * per wave; the result does not depend on how steps are split into
+ * launches. */
 int yh_step(yh_plan* plan, int nsteps, void* stream);
 /* Steps taken since the last yh_set_state (param.count, main.cu:884). */
 int64_t yh_step_count(const yh_plan* plan);
