@@ -253,6 +253,52 @@ def test_tiled_results_do_not_depend_on_launch_split(torch_cuda, model):
             np.testing.assert_array_equal(x, y)
 
 
+def test_random_grids_shapes_and_splits_agree(torch_cuda):
+    """24 seeded cases: grid sides 5..400 (ragged against every tile), model,
+    mask and hole count, a one-column shape, K and a split of the steps into
+    one to three calls all drawn at random.  Each run equals, bit for bit,
+    the same problem stepped one substep per launch on 4 x 4 regions, and
+    stays within 2e-5 of the one-step kernel (which sums the stencil in
+    another order) except where FK's sharp Heaviside flips on a threshold
+    that u meets to within FP32 rounding (at most 1 cell in 1000)."""
+    rng = np.random.default_rng(20261202)
+    shapes = [(4, 4), (8, 4), (16, 4), (4, 8), (8, 8), (16, 8), (4, 16), (8, 16), (7, 16)]
+    for case in range(24):
+        nx, ny = (int(x) for x in rng.integers(5, 401, size=2))
+        model = MODELS[int(rng.integers(2))]
+        masked = bool(rng.integers(2))
+        r, w = shapes[int(rng.integers(len(shapes)))]
+        kmax = min(12, (r * w - 1) // 2, min(nx, ny) - 1)
+        K = int(rng.integers(1, kmax + 1))
+        steps = int(rng.integers(1, 41))
+        cuts = sorted(int(x) for x in rng.integers(0, steps + 1, size=int(rng.integers(0, 3))))
+        calls = [b - a for a, b in zip([0] + cuts, cuts + [steps]) if b > a]
+        print(f"case {case}: {nx}x{ny} model {model} masked {masked} shape {r}x{w} K {K} "
+              f"calls {calls}", flush=True)
+        p = T.grid_params(nx, ny, model=model, solidSwitch=int(masked))
+        mask = (Y.make_holes(nx, ny, int(rng.integers(0, 6)), 1.0, 4.0, Y.seed(7, case, 1))
+                if masked else None)
+        f = Y.init_random(nx, ny, Y.seed(7, case, 2), gate=1.0, nvars=T.nvars(p))
+        out = []
+        for cfg, parts in (((T.YH_KERNEL_TILED, K, r, w, 1), calls),
+                           ((T.YH_KERNEL_TILED, 1, 4, 4, 1), [steps]),
+                           ((T.YH_KERNEL_DIRECT, 1, 0, 0, 1), [steps])):
+            pl = Y.Plan(p)
+            if masked:
+                pl.set_mask(mask)
+            pl.set_state(*f)
+            pl.set_kernel(*cfg)
+            pl.set_option(T.YH_OPT_KEEP_PREV, 0)
+            for c in parts:
+                pl.step(c)
+            out.append(pl.get_state())
+            pl.close()
+        for a in range(len(out[0])):
+            np.testing.assert_array_equal(out[0][a], out[1][a])
+            d = np.abs(out[0][a] - out[2][a])
+            assert np.isfinite(out[0][a]).all() and np.mean(d > 2e-5) < 1e-3, (a, d.max())
+
+
 def test_remainder_merge_is_bitwise_equal(torch_cuda):
     """A 100-step call with K = 9 on 7 rows x 16 waves ends with one launch of
     10 substeps on 8 rows x 16 waves (the remainder merged into the last full
