@@ -208,10 +208,17 @@ static int batch_worklist(yh_batch* b, int K, yh_batch::WL** out) {
   yh_batch::WL w;
   w.cfg = b->cfg;
   w.nitems = (int)items.size();
-  YH_HIP(hipMalloc(&w.tissues, sizeof(BatchTissue) * b->nt));
-  YH_HIP(hipMalloc(&w.items, sizeof(int2) * items.size()));
-  YH_HIP(hipMemcpy(w.tissues, tab.data(), sizeof(BatchTissue) * b->nt, hipMemcpyHostToDevice));
-  YH_HIP(hipMemcpy(w.items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice));
+  hipError_t e = hipMalloc(&w.tissues, sizeof(BatchTissue) * b->nt);
+  if (e == hipSuccess) e = hipMalloc(&w.items, sizeof(int2) * items.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(w.tissues, tab.data(), sizeof(BatchTissue) * b->nt, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(w.items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {  // nothing half-built is kept
+    if (w.tissues) (void)hipFree(w.tissues);
+    if (w.items) (void)hipFree(w.items);
+    return hip_fail(e, "batch work list");
+  }
   b->wl[K] = w;
   *out = &b->wl[K];
   return YH_OK;
