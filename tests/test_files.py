@@ -6,12 +6,14 @@ loadParamValues reads the value column back by position (saveFiles.cu:596-708).
 The positions and labels below are the ones those two functions share; the
 values are checked against printf formatting done here in Python.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from yolohtli_amd import abi_types as T
 from yolohtli_amd import files as FF
-from yolohtli_amd._lib import YhError
+from yolohtli_amd._lib import YhError, lib
 
 # (line, label) of the fields loadParamValues assigns to solver parameters
 SOLVER_LINES = [
@@ -93,6 +95,24 @@ def test_params_csv_short_file_is_an_error(tmp_path):
     path.write_text("a,1\nb,2\n")
     with pytest.raises(YhError):
         FF.loadParamValues(path)
+
+
+@pytest.mark.parametrize("value", ["nan", "1e20", "-inf"])
+def test_params_csv_bad_integer_field_is_an_error(tmp_path, value):
+    """An integer field that is not a finite value in int range is rejected,
+    and the caller's parameters stay as they were."""
+    path = tmp_path / "dataparamcsv.csv"
+    FF.printParameters(path, T.default_params())
+    lines = path.read_text().splitlines()
+    k = next(i for i, l in enumerate(lines) if l.startswith("# grid points X ="))
+    lines[k] = "# grid points X =," + value
+    path.write_text("\n".join(lines) + "\n")
+    q = T.default_params(model=T.YH_MODEL_3V_FK)
+    q.nx = 77
+    with pytest.raises(YhError):
+        FF.loadParamValues(path, q)
+    assert lib().yh_params_read_csv(str(path).encode(), C.byref(q)) == T.YH_EINVAL
+    assert q.nx == 77 and q.model == T.YH_MODEL_3V_FK
 
 
 def test_state_dat_round_trip(tmp_path):

@@ -201,36 +201,49 @@ extern "C" int yh_params_read_csv(const char* path, yh_params* p) {
     v.push_back(comma ? std::strtof(comma + 1, nullptr) : 0.0f);
   }
   if ((int)v.size() < L_COUNT) return YH_EINVAL;
-  const auto I = [&](int l) { return (int)v[l]; };
+  // integer fields must be finite and in int range (the conversion of any
+  // other float is undefined); such a file is rejected
+  bool bad = false;
+  const auto I = [&](int l) {
+    const float x = v[l];
+    if (!(x > -2147483520.0f && x < 2147483520.0f)) {
+      bad = true;
+      return 0;
+    }
+    return (int)x;
+  };
   const auto D = [&](int l) { return (double)v[l]; };
-  p->reduceSym = I(L_REDUCESYM);
-  p->solidSwitch = I(L_SOLID);
-  p->neumannBC = I(L_NEUMANN);
-  p->gateDiff = I(L_GATEDIFF);
-  p->anisotropy = I(L_ANISO);
-  p->lap4 = I(L_LAP4);
-  p->timeIntOrder = I(L_TIMEINT);
-  p->nx = I(L_NX);
-  p->ny = I(L_NY);
-  p->Lx = D(L_LX);
-  p->Ly = D(L_LY);
-  p->dt = D(L_DT);
-  p->diff_par = D(L_DPAR);
-  p->diff_per = D(L_DPER);
-  p->degrad = D(L_DEGRAD);
-  p->rscale = D(L_RSCALE);
-  p->tipOffsetX = I(L_TIPOFFX);
-  p->tipOffsetY = I(L_TIPOFFY);
-  p->boundaryVal = D(L_BVAL);
-  p->Uth = D(L_UTH);
-  p->tc = D(L_TC);
-  p->alpha = D(L_ALPHA);
-  p->beta = D(L_BETA);
-  p->gamma = D(L_GAMMA);
-  p->delta = D(L_DELTA);
-  p->eps = D(L_EPS);
-  p->mu = D(L_MU);
-  p->theta = D(L_THETA);
+  yh_params q = *p;  // *p changes only when the whole file is valid
+  q.reduceSym = I(L_REDUCESYM);
+  q.solidSwitch = I(L_SOLID);
+  q.neumannBC = I(L_NEUMANN);
+  q.gateDiff = I(L_GATEDIFF);
+  q.anisotropy = I(L_ANISO);
+  q.lap4 = I(L_LAP4);
+  q.timeIntOrder = I(L_TIMEINT);
+  q.nx = I(L_NX);
+  q.ny = I(L_NY);
+  q.Lx = D(L_LX);
+  q.Ly = D(L_LY);
+  q.dt = D(L_DT);
+  q.diff_par = D(L_DPAR);
+  q.diff_per = D(L_DPER);
+  q.degrad = D(L_DEGRAD);
+  q.rscale = D(L_RSCALE);
+  q.tipOffsetX = I(L_TIPOFFX);
+  q.tipOffsetY = I(L_TIPOFFY);
+  q.boundaryVal = D(L_BVAL);
+  q.Uth = D(L_UTH);
+  q.tc = D(L_TC);
+  q.alpha = D(L_ALPHA);
+  q.beta = D(L_BETA);
+  q.gamma = D(L_GAMMA);
+  q.delta = D(L_DELTA);
+  q.eps = D(L_EPS);
+  q.mu = D(L_MU);
+  q.theta = D(L_THETA);
+  if (bad) return YH_EINVAL;
+  *p = q;
   return YH_OK;
 }
 
