@@ -289,9 +289,10 @@ def symmetry_reduction(a):
 
 
 def control_grid(a, cfg):
-    """Non-cache-resident control: the headline kernel and the one-step
-    kernel on a --control-n^2 grid (FK, holes at the headline density), whose
-    state exceeds the 256 MB Infinity Cache; launch time by HIP events."""
+    """Non-cache-resident control: the headline kernel, the 8 x 8 shape at
+    K=8 and the one-step kernel on a --control-n^2 grid (FK, holes at the
+    headline density), whose state exceeds the 256 MB Infinity Cache; launch
+    time by HIP events."""
     import torch
     import yolohtli_amd as Y
     from yolohtli_amd import abi_types as T
@@ -303,7 +304,10 @@ def control_grid(a, cfg):
     pl.set_state(*Y.init_random(n, n, Y.seed(1, 1, 2), gate=1.0, nvars=3))
     stream = torch.cuda.current_stream()
     out = {"n": n, "holes": holes, "state_bytes": n * n * 25}
-    for name, c in (("tiled", cfg), ("one_step", (T.YH_KERNEL_DIRECT, 1, 0, 0, 1))):
+    # the headline configuration, 8 rows x 8 waves (two workgroups per CU,
+    # which pipeline better once every CU has many tiles) and the one-step kernel
+    for name, c in (("tiled", cfg), ("tiled_8x8", (T.YH_KERNEL_TILED, 8, 8, 8, 1)),
+                    ("one_step", (T.YH_KERNEL_DIRECT, 1, 0, 0, 1))):
         pl.set_kernel(*c)
         t_ms, K, alg, ach = kernel_timing(a, pl, c, stream, n * n, 25, launches=50)
         out[name] = {"K": K, "rows": c[2], "waves": c[3], "cols": c[4],
