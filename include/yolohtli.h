@@ -161,7 +161,9 @@ int yh_contour_dev(int nx, int ny, int mode, double th1, double th2, double th3,
 /* ------------------------------------------------------------------ plans */
 typedef struct yh_plan yh_plan;
 
-/* Kernel choice for the plain stepper. */
+/* Kernel choice for the plain stepper.  AUTO picks a tiled shape and K from
+ * the grid size and the device's CU count without timing anything;
+ * yh_autotune times the candidates and sets the fastest. */
 enum { YH_KERNEL_AUTO = 0, YH_KERNEL_DIRECT = 1, YH_KERNEL_TILED = 2 };
 typedef struct yh_kernel_cfg {
   int variant;   /* YH_KERNEL_* */

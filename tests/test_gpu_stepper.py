@@ -327,6 +327,54 @@ def test_remainder_merge_is_bitwise_equal(torch_cuda):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+def test_autotune_keeps_the_state_and_installs_its_choice(torch_cuda):
+    """yh_autotune times candidates on the plan's state: the state and the
+    step count come back bit for bit, the returned configuration is the one
+    installed, and stepping on from there equals a plan that was set to that
+    configuration directly.  The tip test needs a new step first (the
+    buffer of the previous state held candidates' outputs)."""
+    n = 256
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    mask = Y.make_holes(n, n, 10, 3.0, 9.0, Y.seed(3, 0, 1))
+    f = Y.init_random(n, n, Y.seed(3, 0, 2), gate=1.0, nvars=3)
+    out = []
+    for tune in (True, False):
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        pl.step(37)
+        if tune:
+            before = pl.get_state()
+            cfg, ms = pl.autotune(steps=64)
+            assert ms > 0
+            for x, y in zip(before, pl.get_state()):
+                np.testing.assert_array_equal(x, y)
+            assert pl.step_count() == 37
+            got = pl.get_kernel()
+            assert (got.variant, got.substeps, got.rows, got.waves, got.cols) == cfg
+            with pytest.raises(RuntimeError):
+                pl.track_tips()
+        else:
+            pl.set_kernel(*cfg)
+        pl.step(10)
+        pl.track_tips()
+        out.append(pl.get_state())
+        pl.close()
+    for x, y in zip(*out):
+        np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("n,want", [(256, (8, 4, 16)), (1024, (9, 7, 16)), (4096, (8, 8, 8))])
+def test_auto_kernel_follows_the_grid(torch_cuda, n, want):
+    """Without timing, AUTO takes 64 x 112 regions at K=9 when their tiles
+    fill between half and all of the 256 CUs (1024^2: 253), 8 x 8 waves at
+    K=8 when there are more tiles than CUs (4096^2), else 4 x 16 at K=8."""
+    pl = Y.Plan(T.grid_params(n, n, model=T.YH_MODEL_3V_FK))
+    c = pl.get_kernel()
+    pl.close()
+    assert (c.variant, c.substeps, c.rows, c.waves, c.cols) == (T.YH_KERNEL_TILED, *want, 1)
+
+
 def test_check_finite_and_argument_errors(torch_cuda):
     p = T.grid_params(64, 64)
     pl = Y.Plan(p)

@@ -62,12 +62,24 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
       return YH_OK;
     }
   }
-  // Save the state, run the candidates, restore.
+  // Save the state, run the candidates, restore.  Everything that can fail
+  // is acquired before the plan is touched.
   const size_t bytes = pl->n * sizeof(float);
   float* save[3] = {nullptr, nullptr, nullptr};
-  for (int a = 0; a < pl->nv; ++a) {
-    YH_HIP(hipMalloc(&save[a], bytes));
-    YH_HIP(hipMemcpyAsync(save[a], pl->buf[pl->cur][a], bytes, hipMemcpyDeviceToDevice, s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  for (int a = 0; a < pl->nv && he == hipSuccess; ++a) {
+    he = hipMalloc(&save[a], bytes);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(save[a], pl->buf[pl->cur][a], bytes, hipMemcpyDeviceToDevice, s);
+  }
+  if (he != hipSuccess) {
+    for (float* q : save)
+      if (q) (void)hipFree(q);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return hip_fail(he, "yh_autotune");
   }
   const int64_t count0 = pl->count;
   const bool keep0 = pl->keep_prev, prev0 = pl->have_prev;
@@ -76,9 +88,6 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
   const bool apd0 = pl->apd_on;  // the timed steps must not touch the APD state
   std::vector<yh_kernel_cfg> cands = candidates(pl);
   pl->apd_on = false;
-  hipEvent_t e0, e1;
-  YH_HIP(hipEventCreate(&e0));
-  YH_HIP(hipEventCreate(&e1));
   yh_kernel_cfg bestc = cfg0;
   float bestt = 1e30f;
   int rc = YH_OK;

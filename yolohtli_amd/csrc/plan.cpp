@@ -12,14 +12,30 @@
 
 namespace yh {
 
+// Tiles of one launch of K substeps on a rows x waves x cols shape.
+static int64_t tile_count(int nx, int ny, int rows, int waves, int cols, int K) {
+  const int wout = 64 * cols - 2 * K, hout = rows * waves - 2 * K;
+  if (wout <= 0 || hout <= 0) return INT64_MAX;
+  return (int64_t)((nx + wout - 1) / wout) * ((ny + hout - 1) / hout);
+}
+
+// YH_KERNEL_AUTO without timing anything (yh_autotune times the candidates):
+// 64 x 112 regions at K=9 when their tiles fill between half and all of the
+// CUs, one each (the best shape measured at 1024^2); 64 x 64 regions at K=8
+// when there are more tiles than CUs, so two workgroups per CU overlap one's
+// loads and stores with the other's substeps (the best measured at 4096^2);
+// else 64 x 64 regions in 16 waves of 4 rows at K=8.
 yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
   yh_kernel_cfg c = pl->cfg;
   if (c.variant == YH_KERNEL_AUTO) {
-    c.variant = YH_KERNEL_TILED;
-    c.substeps = 8;
-    c.rows = 4;
-    c.waves = 16;
-    c.cols = 1;
+    const int64_t t716 = tile_count(pl->p.nx, pl->p.ny, 7, 16, 1, 9);
+    const int64_t cus = pl->cus > 0 ? pl->cus : 256;
+    if (t716 > cus)
+      c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 8, 8, 1};
+    else if (2 * t716 >= cus)
+      c = yh_kernel_cfg{YH_KERNEL_TILED, 9, 7, 16, 1};
+    else
+      c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 4, 16, 1};
   }
   if (c.variant == YH_KERNEL_TILED) {
     const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
@@ -31,13 +47,6 @@ yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
     }
   }
   return c;
-}
-
-// Tiles of one launch of K substeps on a rows x waves x cols shape.
-static int64_t tile_count(int nx, int ny, int rows, int waves, int cols, int K) {
-  const int wout = 64 * cols - 2 * K, hout = rows * waves - 2 * K;
-  if (wout <= 0 || hout <= 0) return INT64_MAX;
-  return (int64_t)((nx + wout - 1) / wout) * ((ny + hout - 1) / hout);
 }
 
 // Whether a remainder of r < K steps can be merged into the last K-step
@@ -186,6 +195,8 @@ int yh_plan_create(const yh_params* p, yh_plan** out) {
   pl->n = (size_t)p->nx * (size_t)p->ny;
   pl->tipcap = 500000;  // TIPVECSIZE, globalVariables.cuh:12
   hipError_t e = hipGetDevice(&pl->device);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&pl->cus, hipDeviceAttributeMultiprocessorCount, pl->device);
   const size_t bytes = pl->n * sizeof(float);
   for (int b = 0; b < 2 && e == hipSuccess; ++b)
     for (int a = 0; a < pl->nv && e == hipSuccess; ++a) e = hipMalloc(&pl->buf[b][a], bytes);

@@ -14,6 +14,7 @@ struct yh_plan {
   int nv = 2;
   size_t n = 0;
   int device = 0;
+  int cus = 0;  // compute units of that device (the AUTO kernel choice)
   float* buf[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   int cur = 0;             // buf[cur] holds the current state
   bool have_prev = false;  // u one step earlier is known: in prev_u or buf[cur^1][0]

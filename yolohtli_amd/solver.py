@@ -191,6 +191,19 @@ class Plan:
         check(lib().yh_get_kernel(self._h, C.byref(cfg)))
         return cfg
 
+    def step_count(self):
+        """Steps taken since the last set_state (param.count)."""
+        return int(lib().yh_step_count(self._h))
+
+    def autotune(self, steps=400, stream=None):
+        """Times the tiled shapes and depths on the current state (the state
+        and step count are restored) and installs the fastest (yh_autotune).
+        Returns ((variant, K, rows, waves, cols), ms per step)."""
+        cfg, ms = T.YhKernelCfg(), C.c_float()
+        check(lib().yh_autotune(self._h, int(steps), _stream_handle(stream), C.byref(cfg),
+                                C.byref(ms)), "yh_autotune")
+        return (cfg.variant, cfg.substeps, cfg.rows, cfg.waves, cfg.cols), ms.value
+
     def step(self, nsteps, stream=None):
         check(lib().yh_step(self._h, int(nsteps), _stream_handle(stream)), "yh_step")
 
