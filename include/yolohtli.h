@@ -287,7 +287,10 @@ int yh_solve_matrix(const double phi[3], const double I[12], double c[3]);
  * steps one tissue, main.cu:876-885).  All tissues share model and mask mode;
  * nx, ny, D and the ionic constants may differ.  One launch advances every
  * tissue by k steps from a (tissue, tile) work list; yh_batch_step replays a
- * captured hipGraph of the launches when graphs are on (default). */
+ * captured hipGraph of the launches when graphs are on (default).  The
+ * kernel starts as k = 6 on 8 rows x 8 waves, one workgroup per item, with k
+ * capped at the smallest tissue's min(nx, ny) - 1; yh_batch_autotune times
+ * the alternatives. */
 typedef struct yh_batch yh_batch;
 int yh_batch_create(const yh_params* per_tissue, int ntissues, yh_batch** out);
 void yh_batch_destroy(yh_batch* batch);

@@ -102,3 +102,28 @@ def test_batch_autotune_and_errors(torch_cuda):
         b2.step(1)
     with pytest.raises(RuntimeError):
         Batch([params[0], T.copy_params(params[1], model=T.YH_MODEL_2V_REF)])
+
+
+def test_default_depth_is_capped_by_the_smallest_tissue(torch_cuda):
+    """With tissues of side 5 and 6 in the batch the default K (6) is capped
+    at 4, the most substeps a reflected halo is exact for, and every tissue
+    matches the one-step kernel."""
+    sides = (5, 6, 64)
+    params = [T.grid_params(n, n, model=T.YH_MODEL_3V_FK) for n in sides]
+    b = Batch(params)
+    fields = [Y.init_random(n, n, Y.seed(9, t, 2), gate=1.0, nvars=3) for t, n in enumerate(sides)]
+    for t in range(len(sides)):
+        b.set_state(t, *fields[t])
+    cfg, _ = b.get_kernel()
+    assert cfg[1] == 4
+    b.set_graph(False)
+    b.step(20)
+    for t, n in enumerate(sides):
+        pl = Y.Plan(params[t])
+        pl.set_state(*fields[t])
+        pl.set_kernel(T.YH_KERNEL_DIRECT, 1, 0, 0, 1)
+        pl.step(20)
+        for x, y in zip(b.get_state(t), pl.get_state()):
+            assert np.max(np.abs(x - y)) < 2e-5
+        pl.close()
+    b.close()

@@ -144,8 +144,11 @@ struct yh_batch {
   uint8_t* solid_d = nullptr;
   uint8_t* codes_d = nullptr;
   std::vector<char> mask_set;
-  yh_kernel_cfg cfg{YH_KERNEL_TILED, 8, 4, 16, 1};
-  int persistent = 1;
+  // the configuration the autotuner picks for the 256-tissue workload; K is
+  // capped at creation by the smallest tissue (reflected halos are exact for
+  // at most min(nx, ny) - 1 substeps)
+  yh_kernel_cfg cfg{YH_KERNEL_TILED, 6, 8, 8, 1};
+  int persistent = 0;
   bool use_graph = true;
   int64_t count = 0;
   // per-K work lists: K -> (tissue table, items, nitems)
@@ -308,6 +311,8 @@ int yh_batch_create(const yh_params* per_tissue, int ntissues, yh_batch** out) {
   }
   b->total = off;
   b->mask_set.assign(ntissues, 0);
+  for (int t = 0; t < ntissues; ++t)
+    b->cfg.substeps = std::min(b->cfg.substeps, std::min(b->p[t].nx, b->p[t].ny) - 1);
   hipError_t e = hipSuccess;
   for (int q = 0; q < 2 && e == hipSuccess; ++q)
     for (int a = 0; a < b->nv && e == hipSuccess; ++a) {
