@@ -74,6 +74,34 @@ def test_plan_steps_match_oracle(torch_cuda, case):
     pl.close()
 
 
+@pytest.mark.parametrize("model,masked,over", [
+    (T.YH_MODEL_2V_REF, 0, dict(timeIntOrder=4, lap4=1, **ANISO)),
+    (T.YH_MODEL_3V_FK, 1, dict(neumannBC=0, boundaryVal=0.0, **ANISO)),
+    (T.YH_MODEL_3V_FK, 0, dict(timeIntOrder=2)),
+])
+def test_larger_grid_100_steps_match_oracle(torch_cuda, model, masked, over):
+    """The heaviest branch combinations on 512 x 384 (ragged against every
+    block) for 100 steps, with the gates of the 70 x 52 cases."""
+    nx, ny = 512, 384
+    rng = np.random.default_rng(77 + model)
+    p = T.grid_params(nx, ny, model=model, solidSwitch=masked, **over)
+    nv = T.nvars(p)
+    f = list(Y.initGates(nx, ny, u_cols=48, rest=1.0 if model else 0.0,
+                         refr=0.0 if model else 1.0, nvars=nv))
+    f[0] = np.clip(f[0] + 0.05 * rng.random(nx * ny).astype(np.float32), 0, 1)
+    m = Y.make_holes(nx, ny, 20, 3.0, 9.0, 77) if masked else None
+    pl = Y.Plan(p)
+    if masked:
+        pl.set_mask(m)
+    pl.set_state(*f)
+    pl.step(100)
+    got = pl.get_state()
+    pl.close()
+    want = O.run_general(p, [x.astype(np.float64) for x in f], 100, solid=m)
+    print(over, [float(np.abs(g - w).max()) for g, w in zip(got, want)])
+    _gate(model, got, want)
+
+
 @pytest.mark.parametrize("model", [T.YH_MODEL_2V_REF, T.YH_MODEL_3V_FK])
 @pytest.mark.parametrize("over", [dict(), dict(neumannBC=0, boundaryVal=0.2, **ANISO)])
 def test_wrapper_with_mouse_stimulus(torch_cuda, model, over):
