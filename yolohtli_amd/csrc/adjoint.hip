@@ -450,33 +450,52 @@ __global__ __launch_bounds__(kArThreads) void k_adj_rev_tiled(
   }
   for (int o = threadIdx.x; o < kArTX * kArTY; o += kArThreads) s_gd[o] = 0.0;
   double gtd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int s = 0; s < K; ++s) {
-    const int b = s & 1;
-    const int step = n_top - s;
-    // the step's state: segment slot slot_top - s, slot 0 being the checkpoint x0
+  // A region has at most kArRN <= kArThreads cells, so thread t holds region
+  // cell t (u of the step's state) and computes the t-th cell of the step's
+  // shrinking region.  The values a step needs from global memory (u, the
+  // computed cell's v and w, its sensor source) are loaded one step ahead,
+  // during the previous step's arithmetic.
+  static_assert(kArRN <= kArThreads, "one region cell per thread");
+  const int tid = threadIdx.x;
+  struct Ahead {
+    float u = 0.0f, v = 0.0f, w = 0.0f, src = 0.0f;
+  };
+  auto fetch = [&](int s) {
+    Ahead a;
     const float* X = slot_top - s == 0 ? x0 : seg + (size_t)(slot_top - s) * 3 * n;
-    for (int l = threadIdx.x; l < rn; l += kArThreads)
-      s_U[l] = X[(size_t)(gx0 + l % rw) + (size_t)nx * (gy0 + l / rw)];
-    __syncthreads();
-    const float* srow = (src && step >= 1 && step % rec == 0) ? src + (size_t)(step / rec - 1) * ns
-                                                              : nullptr;
+    if (tid < rn) a.u = X[(size_t)(gx0 + tid % rw) + (size_t)nx * (gy0 + tid / rw)];
     const int cx0 = gx0 > 0 ? gx0 + s + 1 : 0, cx1 = gx1 < nx ? gx1 - s - 1 : nx;
     const int cy0 = gy0 > 0 ? gy0 + s + 1 : 0, cy1 = gy1 < ny ? gy1 - s - 1 : ny;
     const int cw = cx1 - cx0, cn = cw * (cy1 - cy0);
-    for (int q = threadIdx.x; q < cn; q += kArThreads) {
-      const int gi = cx0 + q % cw, gj = cy0 + q / cw;
-      const int l = (gi - gx0) + rw * (gj - gy0);
-      const size_t c = (size_t)gi + (size_t)nx * gj;
-      LdsAcc A{s_U, s_D, s_lu[b], s_c, l, rw};
-      float sv = 0.0f;
-      if (srow) {
+    if (tid < cn) {
+      const size_t c = (size_t)(cx0 + tid % cw) + (size_t)nx * (cy0 + tid / cw);
+      a.v = X[n + c];
+      a.w = X[2 * n + c];
+      const int step = n_top - s;
+      if (src && step >= 1 && step % rec == 0) {
         const int sm = smap[c];
-        if (sm >= 0) sv = srow[sm];
+        if (sm >= 0) a.src = src[(size_t)(step / rec - 1) * ns + sm];
       }
+    }
+    return a;
+  };
+  Ahead next = fetch(0);
+  for (int s = 0; s < K; ++s) {
+    const int b = s & 1;
+    const Ahead cur = next;
+    if (tid < rn) s_U[tid] = cur.u;
+    if (s + 1 < K) next = fetch(s + 1);
+    __syncthreads();
+    const int cx0 = gx0 > 0 ? gx0 + s + 1 : 0, cx1 = gx1 < nx ? gx1 - s - 1 : nx;
+    const int cy0 = gy0 > 0 ? gy0 + s + 1 : 0, cy1 = gy1 < ny ? gy1 - s - 1 : ny;
+    const int cw = cx1 - cx0, cn = cw * (cy1 - cy0);
+    if (tid < cn) {
+      const int gi = cx0 + tid % cw, gj = cy0 + tid / cw;
+      const int l = (gi - gx0) + rw * (gj - gy0);
+      LdsAcc A{s_U, s_D, s_lu[b], s_c, l, rw};
       float lu, lv, lw, gt[6];
       double gd;
-      adj_rev_cell(k, A, gi, gj, X[n + c], X[2 * n + c], s_lv[l], s_lw[l], sv, lu, lv, lw, gd,
-                   gt);
+      adj_rev_cell(k, A, gi, gj, cur.v, cur.w, s_lv[l], s_lw[l], cur.src, lu, lv, lw, gd, gt);
       s_lu[b ^ 1][l] = lu;
       s_lv[l] = lv;
       s_lw[l] = lw;
