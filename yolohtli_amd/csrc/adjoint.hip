@@ -205,6 +205,7 @@ __global__ __launch_bounds__(kAfThreads) void k_adj_fwd_tiled(
   __shared__ float s_u[2][kAfRN];
   __shared__ float s_v[kAfRN], s_w[kAfRN], s_D[kAfRN];
   __shared__ uint8_t s_c[kAfRN];
+  __shared__ int s_sm[kAfRN];  // sensor slot of each region cell (-1: none), read once
   const int nx = k.nx, ny = k.ny;
   const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
   const int ox0 = tx * kAfTX, oy0 = ty * kAfTY;
@@ -219,6 +220,7 @@ __global__ __launch_bounds__(kAfThreads) void k_adj_fwd_tiled(
     s_w[l] = in.f[2][c];
     s_D[l] = D[c];
     s_c[l] = codes ? codes[c] : kCodeAllOnes;
+    s_sm[l] = traces ? smap[c] : -1;
   }
   __syncthreads();
   for (int s = 0; s < K; ++s) {
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(kAfThreads) void k_adj_fwd_tiled(
           out.f[2][c] = wn;
         }
         if (row) {
-          const int sm = smap[c];
+          const int sm = s_sm[l];
           if (sm >= 0) row[sm] = un;
         }
       }
