@@ -86,8 +86,11 @@ typedef struct orc_sr_state {
 void orc_sr_init(orc_sr_state* s, float tip0x, float tip0y);
 /* One symmetry-reduction step (main.cu:894-953) in place on X.  Its pieces
  * (RD step, tips, slices, integrals, solve, Cxy, BFECC) are pinned one by one
- * in tests/; the order that composes them is restated from main.cu:894-953
- * and has no check of its own: parity unpinned beyond its parts. */
+ * in tests/, and so is the solve-then-advance order (c is the solve of the
+ * step's own integrals with the phi from before the step; phi then advances
+ * by dt*c).  Which state each remaining piece sees (RD output vs old state
+ * for tips, slices and BFECC) is restated from main.cu:894-953 with no check
+ * of its own: parity unpinned for that part. */
 int orc_sr_step_d(const yh_params* p, const uint8_t* solid, double* const X[3],
                   orc_sr_state* s);
 

@@ -7,12 +7,16 @@
   * BFECC: identity at c = 0, constants preserved, second-order convergence of
     a translating Gaussian;
   * tips: bilinear fields with a known isoline intersection;
-  * APD: linear up/down strokes with known threshold crossing times.
+  * APD: linear up/down strokes with known threshold crossing times;
+  * the SR step's composition (main.cu:894-953): each step's c is the 3x3
+    solve of that step's own integrals with the phi from before the step,
+    and phi then advances by dt times that c (main.cu:935-941).
 """
 import numpy as np
 import pytest
 
 import oracle.oracle as O
+import yolohtli_amd as Y
 from yolohtli_amd import abi_types as T
 
 
@@ -159,3 +163,30 @@ def test_apd_of_linear_strokes():
         O.apd_update(dt, count, prev, u, st)
         prev = u
     np.testing.assert_allclose(st["apd1"], dt * (down_at - up_at), rtol=1e-9)
+
+
+@pytest.mark.parametrize("model", [T.YH_MODEL_2V_REF, T.YH_MODEL_3V_FK])
+def test_sr_step_solves_with_old_phi_then_advances_phi(model):
+    """Over 8 oracle SR steps: c equals solve_matrix(phi before the step, the
+    integrals of the step) bit for bit, and phi after the step equals
+    phi before + dt * c bit for bit.  A composition that solved with the
+    advanced phi, advanced phi with the previous c, or solved with the
+    previous step's integrals fails one of the two."""
+    n = 40
+    p0 = T.grid_params(n, n, model=model)
+    nv = T.nvars(p0)
+    f = [np.asarray(x, np.float64) for x in
+         Y.initGates(n, n, u_cols=5, rest=1.0 if model else 0.0, refr=0.0 if model else 1.0,
+                     nvars=nv)]
+    X = O.run(p0, f, 300)
+    p = T.grid_params(n, n, model=model, reduceSym=1, tipOffsetX=12, tipOffsetY=12)
+    ref = O.SR(p, X, tip0=(n / 2, n / 2))
+    dt = O.derive(p).dt
+    for _ in range(8):
+        phi0 = ref.phi.copy()
+        ref.step(1)
+        I = np.array(ref.s.I[:])
+        c = ref.c
+        assert np.isfinite(c).all() and np.abs(I).max() > 0
+        np.testing.assert_array_equal(c, O.solve_matrix(phi0, I))
+        np.testing.assert_array_equal(ref.phi, phi0 + dt * c)
