@@ -141,7 +141,7 @@ hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields o
 }
 
 // ------------------------------------------------------------------- tiled
-template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV>
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV, int ISO = -1>
 __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, Fields out,
                                                    const uint8_t* __restrict__ codes, int K,
                                                    int tiles_x, int ntiles, const ApdState apd,
@@ -149,8 +149,8 @@ __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, 
   __shared__ TileLds<MODEL, NW, CW> lds;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int ty = tile / tiles_x;
-  tile_advance<MODEL, MASKED, R, NW, CW, APD, PREV>(k, in, out, codes, K, tile - ty * tiles_x, ty,
-                                                    lds, apd, prev);
+  tile_advance<MODEL, MASKED, R, NW, CW, APD, PREV, ISO>(k, in, out, codes, K,
+                                                         tile - ty * tiles_x, ty, lds, apd, prev);
 }
 
 bool tiled_supported(int rows, int waves, int cols) {
@@ -172,7 +172,8 @@ bool tiled_prev_supported(int rows, int waves, int cols) {
   return tiled_apd_supported(rows, waves, cols) || (cols == 1 && rows == 7 && waves == 16);
 }
 
-template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV = false>
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV = false,
+          int ISO = -1>
 static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out,
                                  const uint8_t* codes, hipStream_t s, const ApdState& apd,
                                  float* prev = nullptr) {
@@ -181,8 +182,8 @@ static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out
   const int tiles_x = (k.nx + wout - 1) / wout;
   const int tiles_y = (k.ny + hout - 1) / hout;
   const int ntiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV>), dim3(ntiles), dim3(64 * NW),
-                     0, s, k, in, out, codes, K, tiles_x, ntiles, apd, prev);
+  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV, ISO>), dim3(ntiles),
+                     dim3(64 * NW), 0, s, k, in, out, codes, K, tiles_x, ntiles, apd, prev);
   return hipGetLastError();
 }
 
@@ -202,6 +203,19 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
     YH_TILE_APD(8, 16)
 #undef YH_TILE_APD
     return hipErrorInvalidValue;
+  }
+  // the shapes of the 1024^2 stepper (7 x 16 and the 8 x 16 its remainders
+  // merge into) also come with isotropic diffusion fixed at compile time
+  if (k.iso && cols == 1 && waves == 16 && (rows == 7 || rows == 8)) {
+    if (rows == 7)
+      return prev ? launch_tiled_t<MODEL, MASKED, 7, 16, 1, false, true, 1>(K, k, in, out, codes,
+                                                                            s, a, prev)
+                  : launch_tiled_t<MODEL, MASKED, 7, 16, 1, false, false, 1>(K, k, in, out,
+                                                                             codes, s, a);
+    return prev ? launch_tiled_t<MODEL, MASKED, 8, 16, 1, false, true, 1>(K, k, in, out, codes, s,
+                                                                          a, prev)
+                : launch_tiled_t<MODEL, MASKED, 8, 16, 1, false, false, 1>(K, k, in, out, codes,
+                                                                           s, a);
   }
   if (prev) {
 #define YH_TILE_PREV(RR, WW)                   \

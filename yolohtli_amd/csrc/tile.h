@@ -51,7 +51,10 @@ struct TileLds {
 // the output cells one substep before the end is written to prev_u (0 in
 // holes): the previous state the tip test needs, without a single-step tail.
 // K >= 1 (the launchers reject K < 1).
-template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false, bool PREV = false>
+// ISO: -1 takes k.iso at run time; 1 (0) fixes an isotropic (anisotropic)
+// diffusion at compile time, which drops that branch from the row loop.
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false, bool PREV = false,
+          int ISO = -1>
 __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Fields out,
                                              const uint8_t* __restrict__ codes, int K, int tx,
                                              int ty, TileLds<MODEL, NW, CW>& lds,
@@ -166,7 +169,8 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
     addr_e = t;
   }
   [[maybe_unused]] const typename M::Pinned pinned = M::pin(k);
-  const float dscale = k.iso ? k.trx : 1.0f;  // the laplacian terms below carry trx otherwise
+  const bool iso = ISO < 0 ? k.iso : ISO == 1;
+  const float dscale = iso ? k.trx : 1.0f;  // the laplacian terms below carry trx otherwise
 
   // One substep; called once with s = 0 (peeled, so the loads above are
   // waited for row by row) and then in a loop.
@@ -219,7 +223,7 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
             float lap_or_t;  // iso: sum c*(nb - ctr); otherwise the laplacian term
             if (MASKED) {
               const int rm = MASKED ? r : 0;
-              if (k.iso) {
+              if (iso) {
                 float t = cs[rm][c] * sv;
                 t = __builtin_fmaf(cn[rm][c], nv, t);
                 t = __builtin_fmaf(ce[rm][c], ev, t);
@@ -233,7 +237,7 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
                 lap_or_t = __builtin_fmaf(k.try_, ty, k.trx * tx);
               }
             } else {
-              if (k.iso) {
+              if (iso) {
                 lap_or_t = __builtin_fmaf(-4.0f, ctr, (wv + ev) + (nv + sv));
               } else {
                 const float lx = __builtin_fmaf(-2.0f, ctr, wv + ev);
