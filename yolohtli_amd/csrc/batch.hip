@@ -31,7 +31,9 @@ struct BatchTissue {
   int pad;
 };
 
-template <int MODEL, bool MASKED, int R, int NW, int CW>
+// ISO = 1 when every tissue of the batch has isotropic diffusion (fixed at
+// compile time, as in k_tiled), -1 to take each tissue's flag at run time.
+template <int MODEL, bool MASKED, int R, int NW, int CW, int ISO>
 __global__ __launch_bounds__(64 * NW) void k_batch(const BatchTissue* __restrict__ tissues,
                                                    const int2* __restrict__ items, int nitems,
                                                    CFields in, Fields out,
@@ -57,25 +59,25 @@ __global__ __launch_bounds__(64 * NW) void k_batch(const BatchTissue* __restrict
     Fields to{{out.f[0] + off, out.f[1] + off, out.f[2] ? out.f[2] + off : nullptr}};
     const int ty = item.y / T.tiles_x;
     __syncthreads();  // the previous item's last reads of lds are done
-    tile_advance<MODEL, MASKED, R, NW, CW>(T.k, ti, to, MASKED ? codes + off : nullptr, K,
-                                           item.y - ty * T.tiles_x, ty, lds);
+    tile_advance<MODEL, MASKED, R, NW, CW, false, false, ISO>(
+        T.k, ti, to, MASKED ? codes + off : nullptr, K, item.y - ty * T.tiles_x, ty, lds);
   }
 }
 
-template <int MODEL, bool MASKED, int R, int NW, int CW>
+template <int MODEL, bool MASKED, int R, int NW, int CW, int ISO>
 static hipError_t launch_batch_t(int grid, const BatchTissue* t, const int2* items, int nitems,
                                  CFields in, Fields out, const uint8_t* codes, int K,
                                  hipStream_t s) {
-  hipLaunchKernelGGL((k_batch<MODEL, MASKED, R, NW, CW>), dim3(grid), dim3(64 * NW), 0, s, t,
-                     items, nitems, in, out, codes, K);
+  hipLaunchKernelGGL((k_batch<MODEL, MASKED, R, NW, CW, ISO>), dim3(grid), dim3(64 * NW), 0, s,
+                     t, items, nitems, in, out, codes, K);
   return hipGetLastError();
 }
 
-template <int MODEL, bool MASKED, int R, int NW, int CW>
+template <int MODEL, bool MASKED, int R, int NW, int CW, int ISO>
 static int max_resident(int* out) {
   int per_cu = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, (const void*)k_batch<MODEL, MASKED, R, NW, CW>, 64 * NW, 0);
+      &per_cu, (const void*)k_batch<MODEL, MASKED, R, NW, CW, ISO>, 64 * NW, 0);
   if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   int dev = 0, cus = 0;
   YH_HIP(hipGetDevice(&dev));
@@ -96,11 +98,17 @@ static int max_resident(int* out) {
   } while (0)
 
 #define YH_BATCH_ONE(M, MK, RR, WW, CC, FN, ...) \
-  if (rows == RR && waves == WW && cols == CC) return FN<M, MK, RR, WW, CC>(__VA_ARGS__);
+  if (rows == RR && waves == WW && cols == CC) return FN<M, MK, RR, WW, CC, -1>(__VA_ARGS__);
+
+// the default shape also comes with isotropic diffusion fixed at compile time
+#define YH_BATCH_ISO(M, MK, RR, WW, CC, FN, ...)                  \
+  if (rows == RR && waves == WW && cols == CC)                    \
+    return iso ? FN<M, MK, RR, WW, CC, 1>(__VA_ARGS__)            \
+               : FN<M, MK, RR, WW, CC, -1>(__VA_ARGS__);
 
 #define YH_BATCH_SHAPES(M, MK, FN, ...)            \
   YH_BATCH_ONE(M, MK, 4, 8, 1, FN, __VA_ARGS__)   \
-  YH_BATCH_ONE(M, MK, 8, 8, 1, FN, __VA_ARGS__)   \
+  YH_BATCH_ISO(M, MK, 8, 8, 1, FN, __VA_ARGS__)   \
   YH_BATCH_ONE(M, MK, 4, 16, 1, FN, __VA_ARGS__)  \
   YH_BATCH_ONE(M, MK, 8, 16, 1, FN, __VA_ARGS__)  \
   YH_BATCH_ONE(M, MK, 16, 8, 1, FN, __VA_ARGS__)  \
@@ -116,15 +124,16 @@ static bool batch_shape_ok(int rows, int waves, int cols) {
          (cols == 2 && ((rows == 4 && (waves == 8 || waves == 16)) || (rows == 8 && waves == 8)));
 }
 
-static hipError_t launch_batch(int model, bool masked, int rows, int waves, int cols, int grid,
-                               const BatchTissue* t, const int2* items, int nitems, CFields in,
+static hipError_t launch_batch(int model, bool masked, bool iso, int rows, int waves, int cols,
+                               int grid, const BatchTissue* t, const int2* items, int nitems, CFields in,
                                Fields out, const uint8_t* codes, int K, hipStream_t s) {
   if (K < 1) return hipErrorInvalidValue;
   YH_BATCH_DISPATCH(launch_batch_t, grid, t, items, nitems, in, out, codes, K, s);
   return hipErrorInvalidValue;
 }
 
-static int batch_resident(int model, bool masked, int rows, int waves, int cols, int* out) {
+static int batch_resident(int model, bool masked, bool iso, int rows, int waves, int cols,
+                          int* out) {
   YH_BATCH_DISPATCH(max_resident, out);
   return YH_EINVAL;
 }
@@ -134,6 +143,7 @@ static int batch_resident(int model, bool masked, int rows, int waves, int cols,
 struct yh_batch {
   int nt = 0, model = 0, nv = 2;
   bool masked = false;
+  bool iso = true;  // every tissue has isotropic diffusion
   std::vector<yh_params> p;
   std::vector<yh_derived> d;
   std::vector<yh::KParams> k;
@@ -239,14 +249,14 @@ static int batch_enqueue(yh_batch* b, int nsteps, hipStream_t s) {
     int grid = w->nitems;
     if (b->persistent) {
       int res = 0;
-      rc = batch_resident(b->model, b->masked, c.rows, c.waves, c.cols, &res);
+      rc = batch_resident(b->model, b->masked, b->iso, c.rows, c.waves, c.cols, &res);
       if (rc) return rc;
       grid = std::min(grid, (res + 7) / 8 * 8);
     }
     const int bi = b->cur;
     CFields in{{b->buf[bi][0], b->buf[bi][1], b->buf[bi][2]}};
     Fields out{{b->buf[bi ^ 1][0], b->buf[bi ^ 1][1], b->buf[bi ^ 1][2]}};
-    YH_HIP(launch_batch(b->model, b->masked, c.rows, c.waves, c.cols, grid, w->tissues,
+    YH_HIP(launch_batch(b->model, b->masked, b->iso, c.rows, c.waves, c.cols, grid, w->tissues,
                         w->items, w->nitems, in, out, b->masked ? b->codes_d : nullptr, K, s));
     b->cur ^= 1;
     remaining -= K;
@@ -304,6 +314,7 @@ int yh_batch_create(const yh_params* per_tissue, int ntissues, yh_batch** out) {
     b->p.push_back(p);
     b->d.push_back(d);
     b->k.push_back(k);
+    b->iso = b->iso && k.iso;
     b->off.push_back(off);
     const size_t n = (size_t)p.nx * p.ny;
     b->ncell.push_back(n);
@@ -429,7 +440,8 @@ int yh_batch_step(yh_batch* b, int nsteps, void* stream) {
       if (rc) return rc;
     }
     int res = 0;
-    int rc = batch_resident(b->model, b->masked, b->cfg.rows, b->cfg.waves, b->cfg.cols, &res);
+    int rc = batch_resident(b->model, b->masked, b->iso, b->cfg.rows, b->cfg.waves, b->cfg.cols,
+                            &res);
     if (rc) return rc;
     if (!b->cap_stream) YH_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
     const int cur0 = b->cur;

@@ -204,18 +204,20 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
 #undef YH_TILE_APD
     return hipErrorInvalidValue;
   }
-  // the shapes of the 1024^2 stepper (7 x 16 and the 8 x 16 its remainders
-  // merge into) also come with isotropic diffusion fixed at compile time
-  if (k.iso && cols == 1 && waves == 16 && (rows == 7 || rows == 8)) {
-    if (rows == 7)
-      return prev ? launch_tiled_t<MODEL, MASKED, 7, 16, 1, false, true, 1>(K, k, in, out, codes,
-                                                                            s, a, prev)
-                  : launch_tiled_t<MODEL, MASKED, 7, 16, 1, false, false, 1>(K, k, in, out,
-                                                                             codes, s, a);
-    return prev ? launch_tiled_t<MODEL, MASKED, 8, 16, 1, false, true, 1>(K, k, in, out, codes, s,
-                                                                          a, prev)
-                : launch_tiled_t<MODEL, MASKED, 8, 16, 1, false, false, 1>(K, k, in, out, codes,
+  // the shapes the automatic choice takes (7 x 16 and the 8 x 16 its
+  // remainders merge into on 1024^2, 8 x 8 on grids with more tiles than CUs)
+  // also come with isotropic diffusion fixed at compile time
+  if (k.iso && cols == 1) {
+#define YH_TILE_ISO(RR, WW)                                                                     \
+  if (rows == RR && waves == WW)                                                                \
+    return prev ? launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, true, 1>(K, k, in, out, codes, \
+                                                                          s, a, prev)            \
+                : launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, false, 1>(K, k, in, out, codes, \
                                                                            s, a);
+    YH_TILE_ISO(7, 16)
+    YH_TILE_ISO(8, 16)
+    YH_TILE_ISO(8, 8)
+#undef YH_TILE_ISO
   }
   if (prev) {
 #define YH_TILE_PREV(RR, WW)                   \
