@@ -85,6 +85,46 @@ def test_tiled_configs_match_direct(torch_cuda, model, masked):
 
 
 @pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("masked", [False, True])
+def test_unequal_spacing_tiled_matches_oracle_and_direct(torch_cuda, model, masked):
+    """hy = 1.3 hx, so the x and y stencil weights differ (trx != try): the
+    shapes that also exist with isotropic weights fixed at compile time
+    (7 x 16, 8 x 16, 8 x 8) must take the general weights here.  One step
+    against the FP64 oracle (2e-6, as the single-step test), 23 steps against
+    the direct kernel (2e-5, as the tiled configurations)."""
+    rng = np.random.default_rng(31 + model + 2 * masked)
+    p = T.grid_params(131, 77, model=model, solidSwitch=int(masked))
+    p.Ly = 1.3 * p.Ly
+    c = O.derive(p)
+    assert abs(c.rx / c.ry - 1.69) < 1e-9
+    f = _fields(rng, p)
+    solid = (rng.random(p.nx * p.ny) > 0.25).astype(np.uint8) if masked else None
+    shapes = [(9, 7, 16, 1), (3, 8, 16, 1), (8, 8, 8, 1), (4, 4, 16, 1), (4, 8, 8, 2)]
+
+    def run(cfg, steps):
+        pl = Y.Plan(p)
+        if masked:
+            pl.set_mask(solid)
+        pl.set_kernel(*cfg)
+        pl.set_state(*f)
+        pl.step(steps)
+        out = pl.get_state()
+        pl.close()
+        return out
+
+    one, _ = O.rd_step(p, [x.astype(np.float64) for x in f], solid)
+    want = run((T.YH_KERNEL_DIRECT,), 23)
+    for K, r, w, cc in shapes:
+        got1 = run((T.YH_KERNEL_TILED, 1, r, w, cc), 1)
+        got = run((T.YH_KERNEL_TILED, K, r, w, cc), 23)
+        for a in range(len(got)):
+            e1 = np.max(np.abs(got1[a].astype(np.float64) - one[a]))
+            assert e1 < 2e-6, ("one step", r, w, cc, a, e1)
+            err = np.max(np.abs(got[a].astype(np.float64) - want[a]))
+            assert err < 2e-5, (K, r, w, cc, a, err)
+
+
+@pytest.mark.parametrize("model", MODELS)
 def test_512_cross_field_1000_steps_within_1e4(torch_cuda, model):
     nx = ny = 512
     p = T.grid_params(nx, ny, model=model)
