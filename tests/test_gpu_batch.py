@@ -29,10 +29,16 @@ def _batch(params, masks, fields, **cfg):
     return b
 
 
-@pytest.mark.parametrize("shape", [(6, 4, 16, 1), (8, 7, 16, 1)])
+@pytest.mark.parametrize("aniso", [False, True])
+@pytest.mark.parametrize("shape", [(6, 4, 16, 1), (8, 7, 16, 1), (6, 8, 8, 1)])
 @pytest.mark.parametrize("model", [T.YH_MODEL_2V_REF, T.YH_MODEL_3V_FK])
-def test_batch_equals_single_tissue_kernel(torch_cuda, model, shape):
+def test_batch_equals_single_tissue_kernel(torch_cuda, model, shape, aniso):
+    """aniso: one tissue with hy = 1.3 hx, so the batch (8 x 8 has an
+    all-isotropic variant) takes each tissue's weights at run time, while the
+    single-tissue launches of the other tissues keep the isotropic variant."""
     params, masks, fields = _small_workload(model)
+    if aniso:
+        params[1].Ly = 1.3 * params[1].Ly
     K, r, w, c = shape
     cfg = dict(substeps=K, rows=r, waves=w, cols=c)
     b = _batch(params, masks, fields, persistent=1, **cfg)
