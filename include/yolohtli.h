@@ -47,7 +47,10 @@ const char* yh_strerror(int code);
 enum { YH_MODEL_2V_REF = 0, YH_MODEL_3V_FK = 1 };
 
 /* Fenton-Karma three-variable model (Fenton & Karma, Chaos 8:20, 1998).
- * Not present in the reference; defaults are the paper's Beeler-Reuter fit. */
+ * Not present in the reference; defaults are the paper's Beeler-Reuter fit.
+ * v opens with tau_v1m while u < u_v and with tau_v2m while u_v <= u < u_c
+ * (the naming of Fenton et al., Chaos 12:852, 2002; the 1998 paper calls the
+ * same two constants tau_v2- = 19.6 and tau_v1- = 1250). */
 typedef struct yh_fk_params {
   double tau_d, tau_r, tau_si, tau_0; /* ms: fast-in, slow-out, slow-in, outward */
   double tau_vp, tau_v1m, tau_v2m;    /* ms: v gate closing / opening (u<u_v, u>=u_v) */

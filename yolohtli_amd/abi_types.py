@@ -73,9 +73,15 @@ class YhTip(C.Structure):
 
 
 def default_fk():
-    """Fenton-Karma 1998 Beeler-Reuter parameter set (same values as yh_fk_params_default)."""
+    """Fenton-Karma 1998 Beeler-Reuter parameter set (same values as yh_fk_params_default).
+
+    tau_v1m is the v recovery constant below u_v (rest), tau_v2m the one for
+    u_v <= u < u_c: the paper's tau_v2- = 19.6 and tau_v1- = 1250, whose
+    tau_v-(u) = H(u - u_v) tau_v1- + H(u_v - u) tau_v2- names them the other
+    way round (Fenton et al. 2002 use this library's naming).
+    """
     return YhFkParams(tau_d=0.25, tau_r=33.33, tau_si=29.0, tau_0=12.5, tau_vp=3.33,
-                      tau_v1m=1250.0, tau_v2m=19.6, tau_wp=870.0, tau_wm=41.0,
+                      tau_v1m=19.6, tau_v2m=1250.0, tau_wp=870.0, tau_wm=41.0,
                       u_c=0.13, u_v=0.04, u_csi=0.85, k=10.0)
 
 

@@ -114,14 +114,18 @@ const char* yh_strerror(int code) {
 
 void yh_fk_params_default(yh_fk_params* f) {
   if (!f) return;
-  // Fenton & Karma (1998), Table 1, Beeler-Reuter fit.
+  // Fenton & Karma (1998), Table 1, Beeler-Reuter fit.  The paper writes
+  // tau_v-(u) = H(u - u_v) tau_v1- + H(u_v - u) tau_v2- with tau_v1- = 1250,
+  // tau_v2- = 19.6; this library (as Fenton et al., Chaos 12:852, 2002) names
+  // the constant used below u_v tau_v1m, so rest (u < u_v) recovers v with
+  // 19.6 ms and u_v <= u < u_c with 1250 ms, as in the paper.
   f->tau_d = 0.25;  // C_m/g_fi with g_fi = 4
   f->tau_r = 33.33;
   f->tau_si = 29.0;
   f->tau_0 = 12.5;
   f->tau_vp = 3.33;
-  f->tau_v1m = 1250.0;
-  f->tau_v2m = 19.6;
+  f->tau_v1m = 19.6;
+  f->tau_v2m = 1250.0;
   f->tau_wp = 870.0;
   f->tau_wm = 41.0;
   f->u_c = 0.13;
