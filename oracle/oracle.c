@@ -12,6 +12,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -679,10 +680,13 @@ void orc_run_apd_d(const yh_params* p, const uint8_t* solid, double* const st[3]
   const int nv = p->model == YH_MODEL_3V_FK ? 3 : 2;
   double* nw[3] = {0, 0, 0};
   for (int a = 0; a < nv; ++a) nw[a] = (double*)malloc(n * sizeof(double));
+  const int nthreads = omp_get_max_threads();
   for (int s = 0; s < nsteps; ++s) {
     const double* cin[3] = {st[0], st[1], nv == 3 ? st[2] : 0};
-    orc_rd_step_d(p, solid, cin, nw, 0);
+    /* the plain step (no imposed decisions), OpenMP over rows */
+    orc_rd_step_forced_d(p, solid, cin, nw, 0, 0, nthreads);
     const long count = count0 + s + 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (size_t k = 0; k < n; ++k) {
       if (p->solidSwitch && !solid[k]) continue;
       const double uo = st[0][k], un = nw[0][k];

@@ -44,6 +44,15 @@ void orc_run_d(const yh_params* p, const uint8_t* solid, double* const st[3],
 void orc_run_f(const yh_params* p, const uint8_t* solid, float* const st[3],
                int nsteps, int nthreads);
 
+/* orc_rd_step_d with imposed Heaviside decisions (FK): force_p[c] and
+ * force_q[c] = 0 or 1 replace STEP(u - u_c) and STEP(u - u_v) of cell c, -1
+ * keeps the cell's own.  A run that takes another implementation's
+ * decisions where they differ from its own ("shadowing") separates the
+ * effect of threshold ties from that of the update arithmetic. */
+void orc_rd_step_forced_d(const yh_params* p, const uint8_t* solid,
+                          const double* const in[3], double* const out[3],
+                          const int8_t* force_p, const int8_t* force_q, int nthreads);
+
 /* spiralTip_kernel + tipRecordPlane (tipTracker.cu:20-241), row-major cell
  * order, clockwise root before counter-clockwise root.  Returns the number of
  * tips found (entries beyond cap are not written). keys = 2*cell+root. */

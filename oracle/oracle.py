@@ -60,6 +60,7 @@ def lib():
         h.orc_sr_step_d.argtypes = [PP, P, P, C.POINTER(OrcSrState)]
         h.orc_sr_step_d.restype = C.c_int
         h.orc_apd_update_d.argtypes = [C.c_int, C.c_double, C.c_long, P, P, P, P, P, P, P]
+        h.orc_rd_step_forced_d.argtypes = [PP, P, P, P, P, P, C.c_int]
         h.orc_run_apd_d.argtypes = [PP, P, P, C.c_int, C.c_long, C.c_double, P, P, P, P, P]
         h.orc_rd_step_general_d.argtypes = [PP, P, C.c_int, C.c_int, C.c_int, P, P, P]
         h.orc_rd_step_general_d.restype = C.c_int
@@ -118,6 +119,23 @@ def run(p, fields, nsteps, solid=None, dtype=np.float64, nthreads=None):
     fn = lib().orc_run_d if dtype == np.float64 else lib().orc_run_f
     fn(C.byref(p), _p(m), cst, int(nsteps), int(nthreads or os.cpu_count() or 1))
     return fs
+
+
+def rd_step_forced(p, fields, force_p, force_q, solid=None, nthreads=None):
+    """One FP64 Euler step of the FK model with imposed Heaviside decisions:
+    force_p / force_q (int8 per cell) = 0 or 1 replace STEP(u - u_c) /
+    STEP(u - u_v), -1 keeps the cell's own decision."""
+    nv = nvars(p)
+    n = p.nx * p.ny
+    fin, cin = _arr3(fields[:nv], np.float64)
+    out = [np.empty(n) for _ in range(nv)]
+    cout = (C.c_void_p * 3)(*([_p(f) for f in out] + [None] * (3 - nv)))
+    fp = np.ascontiguousarray(force_p, dtype=np.int8).reshape(-1)
+    fq = np.ascontiguousarray(force_q, dtype=np.int8).reshape(-1)
+    m = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+    lib().orc_rd_step_forced_d(C.byref(p), _p(m), cin, cout, _p(fp), _p(fq),
+                               int(nthreads or os.cpu_count() or 1))
+    return out
 
 
 def rd_step_general(p, fields, solid=None, stim=None, veltan=False):

@@ -4,7 +4,9 @@ Tolerances (FP32 GPU vs FP64 oracle on the same FP32 inputs):
   * one step: 2e-6 absolute (FP32 rounding of O(1) values, a few ops);
   * tiled vs direct kernel after 23 steps: 2e-5 absolute;
   * 1000 steps at 512^2 from the cross-field IC: max|du|, max|dv| <= 1e-4
-    (the accuracy gate of the plain stepper).
+    (the accuracy gate of the plain stepper); for the FK model with its sharp
+    Heaviside switches, the gate holds once the threshold decisions the GPU
+    took differently from FP64 are imposed on the oracle (_fk_gate).
 """
 import numpy as np
 import pytest
@@ -124,57 +126,124 @@ def test_unequal_spacing_tiled_matches_oracle_and_direct(torch_cuda, model, mask
             assert err < 2e-5, (K, r, w, cc, a, err)
 
 
+def _fk_shadow(p, f, nsteps, solid=None):
+    """Runs the GPU plan one launch per step (tiled results do not depend on
+    where launches split, which the caller checks) beside an FP64 oracle run
+    that takes the GPU's Heaviside decisions wherever they differ from its
+    own.  The GPU decides STEP(u - u_c) and STEP(u - u_v) on its FP32 u
+    against the FP32 thresholds; a cell whose FP64 u lies within FP32
+    rounding of a threshold can decide the other way one step earlier or
+    later, and one such switch moves v by up to dt/tau_vp ~ 6e-3.  Returns
+    the GPU state, the shadowed FP64 state, the switches (step, cell,
+    threshold, |u_gpu - u_fp64| at that step) and the largest |u_gpu - u_fp64|
+    over all steps and cells."""
+    pl = Y.Plan(p)
+    if solid is not None:
+        pl.set_mask(solid)
+    pl.set_state(*f)
+    x = [a.astype(np.float64) for a in f]
+    ug = f[0]
+    uc, uv = np.float32(p.fk.u_c), np.float32(p.fk.u_v)
+    flips, drift = [], 0.0
+    for s in range(nsteps):
+        pg, qg = ug >= uc, ug >= uv
+        fp = np.where(pg != (x[0] >= p.fk.u_c), pg.astype(np.int8), np.int8(-1))
+        fq = np.where(qg != (x[0] >= p.fk.u_v), qg.astype(np.int8), np.int8(-1))
+        for name, ff in (("u_c", fp), ("u_v", fq)):
+            for c in np.nonzero(ff >= 0)[0]:
+                flips.append((s, int(c), name, abs(float(ug[c]) - x[0][c])))
+        x = O.rd_step_forced(p, x, fp, fq, solid=solid)
+        pl.step(1)
+        g = pl.get_state()
+        ug = g[0]
+        drift = max(drift, float(np.max(np.abs(ug - x[0]))))
+    pl.close()
+    return g, x, flips, drift
+
+
+def _near(cells, centres, nx, r):
+    """Whether each cell lies within Chebyshev distance r of a centre."""
+    if len(cells) == 0:
+        return np.ones(0, bool)
+    if len(centres) == 0:
+        return np.zeros(len(cells), bool)
+    ci, cj = np.asarray(cells) % nx, np.asarray(cells) // nx
+    ki, kj = np.asarray(centres) % nx, np.asarray(centres) // nx
+    d = np.maximum(np.abs(ci[:, None] - ki[None, :]), np.abs(cj[:, None] - kj[None, :]))
+    return d.min(axis=1) <= r
+
+
+def _fk_gate(p, f, nsteps, solid=None, radius=8):
+    """The 1e-4 gate for the FK model (sharp Heaviside switches):
+      * the GPU's result does not depend on one launch per step;
+      * with the GPU's threshold decisions imposed on the FP64 oracle, every
+        variable agrees to 1e-4 everywhere, and u agrees to 1e-4 at every step
+        (so each differing decision was taken where FP64's u lay within the
+        gate of the threshold);
+      * against the FP64 oracle alone, every cell off by more than 1e-4 lies
+        within `radius` cells of a cell whose decision differed.
+    Prints the maxima of both comparisons."""
+    nv = T.nvars(p)
+    g, x, flips, drift = _fk_shadow(p, f, nsteps, solid)
+    pl = Y.Plan(p)
+    if solid is not None:
+        pl.set_mask(solid)
+    pl.set_state(*f)
+    pl.step(nsteps)
+    whole = pl.get_state()
+    pl.close()
+    for a in range(nv):
+        assert np.array_equal(whole[a], g[a]), "one launch per step changed the result"
+    want = O.run(p, [a.astype(np.float64) for a in f], nsteps, solid=solid)
+    shadow = [float(np.max(np.abs(g[a] - x[a]))) for a in range(nv)]
+    plain = [float(np.max(np.abs(g[a] - want[a]))) for a in range(nv)]
+    bad = np.nonzero(np.max([np.abs(g[a] - want[a]) for a in range(nv)], axis=0) > 1e-4)[0]
+    centres = sorted({c for _, c, _, _ in flips})
+    print(f"FK {p.nx}x{p.ny} {nsteps} steps: max|d| vs FP64 {plain}; with the GPU's "
+          f"decisions imposed {shadow}; u drift over the run {drift:.2e}; "
+          f"{len(flips)} differing decisions {flips[:8]}; {len(bad)} cells above 1e-4")
+    assert max(shadow) <= 1e-4, shadow
+    assert drift <= 1e-4, drift
+    near = _near(bad, centres, p.nx, radius)
+    assert np.all(near), [(int(c) % p.nx, int(c) // p.nx) for c in np.asarray(bad)[~near][:10]]
+    return g, want, flips
+
+
 @pytest.mark.parametrize("model", MODELS)
 def test_512_cross_field_1000_steps_within_1e4(torch_cuda, model):
+    """The accuracy gate of the plain stepper at 512^2 from the cross-field
+    start: max|d| <= 1e-4 after 1000 steps against the FP64 oracle; for FK
+    with the threshold decisions separated out (_fk_gate)."""
     nx = ny = 512
     p = T.grid_params(nx, ny, model=model)
     nv = T.nvars(p)
     if model == T.YH_MODEL_2V_REF:
         f = Y.initGates(nx, ny)  # reference IC: u=1 for i<nx/8, v=1 for j>=ny/2
-    else:
-        f = Y.initGates(nx, ny, u_cols=32, rest=1.0, refr=0.0, nvars=3)
-    pl = Y.Plan(p)
-    pl.set_state(*f)
-    pl.step(1000)
-    got = pl.get_state()
-    want = O.run(p, [x.astype(np.float64) for x in f], 1000)
-    errs = [float(np.max(np.abs(got[a] - want[a]))) for a in range(nv)]
-    print("max|d| per variable:", errs)
-    if model == T.YH_MODEL_2V_REF:
+        pl = Y.Plan(p)
+        pl.set_state(*f)
+        pl.step(1000)
+        got = pl.get_state()
+        want = O.run(p, [x.astype(np.float64) for x in f], 1000)
+        errs = [float(np.max(np.abs(got[a] - want[a]))) for a in range(nv)]
+        print("max|d| per variable:", errs)
         assert max(errs) <= 1e-4, errs
     else:
-        # FK's sharp Heaviside: a cell whose u lands on u_c or u_v within FP32
-        # rounding switches one step earlier or later than in FP64 (one switch
-        # moves v by dt/tau_vp ~ 6e-3).  Away from such cells the gate holds.
-        for a in range(nv):
-            d = np.abs(got[a] - want[a])
-            assert np.mean(d > 1e-4) < 2e-4, (a, float(np.mean(d > 1e-4)))
-            assert np.quantile(d, 0.9999) <= 1e-4, (a, float(np.quantile(d, 0.9999)))
+        f = Y.initGates(nx, ny, u_cols=32, rest=1.0, refr=0.0, nvars=3)
+        _fk_gate(p, f, 1000)
 
 
 def test_1024_masked_fk_against_oracle(torch_cuda):
+    """The headline workload's first 200 steps (1024^2, 200 holes, random
+    start) under the same gate; holes stay 0."""
     nx = ny = 1024
     p = T.grid_params(nx, ny, model=T.YH_MODEL_3V_FK, solidSwitch=1)
     mask = Y.make_holes(nx, ny, 200, 3.0, 12.0, Y.seed(1, 0, 1))
     f = Y.init_random(nx, ny, Y.seed(1, 0, 2), gate=1.0, nvars=3)
-    pl = Y.Plan(p)
-    pl.set_mask(mask)
-    pl.set_state(*f)
-    pl.step(200)
-    got = pl.get_state()
-    want32 = O.run(p, f, 200, solid=mask, dtype=np.float32)
-    want64 = O.run(p, [x.astype(np.float64) for x in f], 200, solid=mask)
+    got, want, _ = _fk_gate(p, f, 200, solid=mask)
     hole = mask == 0
     for a in range(3):
         assert np.all(got[a][hole] == 0)
-        d64 = np.abs(got[a] - want64[a])
-        d32 = np.abs(got[a] - want32[a])
-        print(a, "max vs fp64", d64.max(), "vs fp32", d32.max(),
-              "frac>1e-4", float(np.mean(d64 > 1e-4)))
-        # a sharp Heaviside switch can flip in a handful of cells where u sits on
-        # a threshold to within FP32 rounding; everywhere else the update agrees
-        assert np.mean(d64 > 1e-4) < 1e-4
-        assert np.median(d64) < 1e-6
+        assert np.median(np.abs(got[a] - want[a])) < 1e-6
 
 
 def test_tips_match_oracle_and_plan_log(torch_cuda):
@@ -348,7 +417,9 @@ def test_remainder_merge_is_bitwise_equal(torch_cuda):
     n = 512
     p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
     mask = Y.make_holes(n, n, 50, 3.0, 12.0, Y.seed(1, 0, 1))
-    f = Y.initGates(n, n, nvars=3)
+    # FK rest state (v = w = 1) with a refractory half: the broken front
+    # curls into the centred disk the masked tip search covers
+    f = Y.initGates(n, n, rest=1.0, refr=0.0, nvars=3)
     out = []
     for calls in ([100], [9] * 11 + [1]):
         pl = Y.Plan(p)
