@@ -36,9 +36,9 @@ struct AdjK {
   float dt, tc;
   float hw;        // Heaviside width
   float td, tr, tsi, t0, tvp, tv1m, tv2m, twp, twm, uc, uv, ucsi, kk;
-  // reciprocals, rounded once on the host from FP64: 1/hw, 0.5/hw, 1/tau_d,
+  // reciprocals, rounded once on the host from FP64: 1/hw, 2/hw, 1/tau_d,
   // 1/tau_r, 0.5/tau_si, 1/tau_0, 1/tau_v+, 1/tau_w-, 1/tau_w+
-  float ihw, hihw, itd, itr, ihtsi, it0, itvp, itwm, itwp;
+  float ihw, tihw, itd, itr, ihtsi, it0, itvp, itwm, itwp;
 };
 
 struct FkEval {
@@ -48,51 +48,63 @@ struct FkEval {
   float dGv_dtvp, dGw_dtwp;
 };
 
-// tanh(x) = 1 - 2/(exp(2x) + 1) with the hardware exp2 and reciprocal:
-// absolute error ~1e-7, exact limits +-1 (exp2 saturates to inf or 0).
-__device__ __forceinline__ float tanh_fast(float x) {
+// s = sigmoid(2y) = 0.5*(1 + tanh y) and its complement sc = 1 - s, both
+// from a = exp(-2|y|) in (0, 1] with the hardware exp2 and reciprocal:
+// the larger is 1/(1 + a), the smaller a/(1 + a), so each keeps a relative
+// error of a few ulp (1 - 2/(exp(2y) + 1) loses the smaller one, and
+// 1 - tanh^2, to cancellation: 2e-3 relative at |y| = 5).
+__device__ __forceinline__ void sigmoid2(float y, float* s, float* sc) {
 #pragma clang fp contract(off)
-  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2/ln 2
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+  const float a = __builtin_amdgcn_exp2f(-2.8853900817779268f * __builtin_fabsf(y));  // 2/ln 2
+  const float big = __builtin_amdgcn_rcpf(1.0f + a);
+  const float small = a * big;
+  *s = y >= 0.0f ? big : small;
+  *sc = y >= 0.0f ? small : big;
 }
 
-__device__ __forceinline__ void heav(float x, float iw, float hiw, float* h, float* dh) {
+// Smoothed Heaviside h = 0.5*(1 + tanh(x/w)), its complement hc = 1 - h and
+// its derivative dh = (1 - tanh^2)/(2w) = (2/w) h (1 - h).
+__device__ __forceinline__ void heav(float x, float iw, float tiw, float* h, float* hc,
+                                     float* dh) {
 #pragma clang fp contract(off)
-  const float t = tanh_fast(x * iw);
-  *h = 0.5f * (1.0f + t);
-  *dh = hiw * (1.0f - t * t);
+  sigmoid2(x * iw, h, hc);
+  *dh = tiw * (*h * *hc);
 }
 
 // Fenton & Karma 1998 currents and derivatives (smoothed Heaviside).  No
 // FMA contraction here or in adj_fwd_cell: every operation rounds on its
-// own, so the kernels that inline them agree bit for bit.
+// own, so the kernels that inline them agree bit for bit.  pc = 1 - p and
+// qc = 1 - q come from heav(); 1 + tanh(k(u - u_csi)) = 2 ss and
+// 1 - tanh^2 = 4 ss ssc with (ss, ssc) = sigmoid2(k(u - u_csi)).
 __device__ FkEval fk_eval(const AdjK& k, float u, float v, float w) {
 #pragma clang fp contract(off)
   FkEval r;
-  float p, dp, q, dq;
-  heav(u - k.uc, k.ihw, k.hihw, &p, &dp);
-  heav(u - k.uv, k.ihw, k.hihw, &q, &dq);
+  float p, pc, dp, q, qc, dq, ss, ssc;
+  heav(u - k.uc, k.ihw, k.tihw, &p, &pc, &dp);
+  heav(u - k.uv, k.ihw, k.tihw, &q, &qc, &dq);
   const float f = (1.0f - u) * (u - k.uc);
   const float df = 1.0f + k.uc - 2.0f * u;
   const float Jfi = -v * p * f * k.itd;
-  const float Jso = u * (1.0f - p) * k.it0 + p * k.itr;
-  const float th = tanh_fast(k.kk * (u - k.ucsi));
-  const float Jsi = -w * (1.0f + th) * k.ihtsi;
+  const float Jso = u * pc * k.it0 + p * k.itr;
+  sigmoid2(k.kk * (u - k.ucsi), &ss, &ssc);
+  const float opth = 2.0f * ss;           // 1 + tanh
+  const float sech2 = 4.0f * (ss * ssc);  // 1 - tanh^2
+  const float Jsi = -w * opth * k.ihtsi;
   r.I = Jfi + Jso + Jsi;
-  r.dIdu = -v * (dp * f + p * df) * k.itd + (1.0f - p) * k.it0 - u * dp * k.it0 + dp * k.itr -
-           w * k.kk * (1.0f - th * th) * k.ihtsi;
+  r.dIdu = -v * (dp * f + p * df) * k.itd + pc * k.it0 - u * dp * k.it0 + dp * k.itr -
+           w * k.kk * sech2 * k.ihtsi;
   r.dIdv = -p * f * k.itd;
-  r.dIdw = -(1.0f + th) * k.ihtsi;
-  const float tvm = (1.0f - q) * k.tv1m + q * k.tv2m;
+  r.dIdw = -opth * k.ihtsi;
+  const float tvm = qc * k.tv1m + q * k.tv2m;
   const float itvm = 1.0f / tvm;
   const float dtvm = dq * (k.tv2m - k.tv1m);
-  const float A = (1.0f - p) * (1.0f - v) * itvm - p * v * k.itvp;
-  const float dAdu = -dp * (1.0f - v) * itvm - (1.0f - p) * (1.0f - v) * dtvm * (itvm * itvm) -
+  const float A = pc * (1.0f - v) * itvm - p * v * k.itvp;
+  const float dAdu = -dp * (1.0f - v) * itvm - pc * (1.0f - v) * dtvm * (itvm * itvm) -
                      dp * v * k.itvp;
-  const float dAdv = -(1.0f - p) * itvm - p * k.itvp;
-  const float B = (1.0f - p) * (1.0f - w) * k.itwm - p * w * k.itwp;
+  const float dAdv = -pc * itvm - p * k.itvp;
+  const float B = pc * (1.0f - w) * k.itwm - p * w * k.itwp;
   const float dBdu = -dp * (1.0f - w) * k.itwm - dp * w * k.itwp;
-  const float dBdw = -(1.0f - p) * k.itwm - p * k.itwp;
+  const float dBdw = -pc * k.itwm - p * k.itwp;
   r.Gv = -A;
   r.Gw = -B;
   r.dGvdu = -dAdu;
@@ -102,7 +114,7 @@ __device__ FkEval fk_eval(const AdjK& k, float u, float v, float w) {
   r.dI_dtau[0] = -Jfi * k.itd;
   r.dI_dtau[1] = -p * (k.itr * k.itr);
   r.dI_dtau[2] = -2.0f * Jsi * k.ihtsi;
-  r.dI_dtau[3] = -u * (1.0f - p) * (k.it0 * k.it0);
+  r.dI_dtau[3] = -u * pc * (k.it0 * k.it0);
   r.dGv_dtvp = -p * v * (k.itvp * k.itvp);
   r.dGw_dtwp = -p * w * (k.itwp * k.itwp);
   return r;
@@ -632,7 +644,7 @@ static void adj_k(yh_adjoint* a, const double tau6[6]) {
   k.ucsi = (float)f.u_csi;
   k.kk = (float)f.k;
   k.ihw = (float)(1.0 / p.heaviside_width);
-  k.hihw = (float)(0.5 / p.heaviside_width);
+  k.tihw = (float)(2.0 / p.heaviside_width);
   k.itd = (float)(1.0 / tau6[0]);
   k.itr = (float)(1.0 / tau6[1]);
   k.ihtsi = (float)(0.5 / tau6[2]);
