@@ -37,7 +37,8 @@ enum {
   YH_ENOTIP = 7,       /* no spiral tip found (informational) */
   YH_EOVERFLOW = 8,    /* tip log saturated; later tips were dropped */
   YH_ENONFINITE = 9,   /* NaN/Inf found in the state */
-  YH_ESTATE = 10       /* call made in the wrong order */
+  YH_ESTATE = 10,      /* call made in the wrong order */
+  YH_ESCRATCH = 11     /* grid larger than yh_step_noalloc's fixed scratch */
 };
 
 /* Human-readable text for an error code. */
@@ -200,6 +201,20 @@ int yh_get_kernel(const yh_plan* plan, yh_kernel_cfg* cfg);
  * per wave; the result does not depend on how steps are split into
  * launches. */
 int yh_step(yh_plan* plan, int nsteps, void* stream);
+/* yh_step without any device allocation, on caller-owned device fields (no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory, such as code under HIP graph capture, tight serving loops and
+ * harnesses that forbid hipMalloc).  Advances nsteps explicit-Euler steps
+ * (standard branch only, else YH_EUNSUPPORTED) from in[] into out[] (never
+ * aliasing; in[] is not written) with the kernel a plan on YH_KERNEL_AUTO
+ * runs, so out[] equals yh_step's state bit for bit.  solid: device mask,
+ * required when solidSwitch.  The stencil codes and the intermediate state
+ * live in a fixed device array of the library, one per device for the
+ * process's lifetime: grids above 2^21 cells return YH_ESCRATCH, and calls
+ * that overlap on one device race (order them, e.g. on one stream).  No tip
+ * or APD output; nsteps = 0 copies in[] to out[]. */
+int yh_step_noalloc(const yh_params* p, float* const out[3], const float* const in[3],
+                    const uint8_t* solid, int nsteps, void* stream);
 /* Steps taken since the last yh_set_state (param.count, main.cu:884). */
 int64_t yh_step_count(const yh_plan* plan);
 /* Detect tips between the previous and the current state and append them to

@@ -32,6 +32,7 @@ PROTOTYPES = {
     "yh_make_holes": (I, [I, I, I, D, D, C.c_uint64, P]),
     "yh_rd_step_dev": (I, [PARAMS_P, P, P, P, P, P]),
     "yh_rd_step_dev_ex": (I, [PARAMS_P, P, P, P, P, I, I, I, P]),
+    "yh_step_noalloc": (I, [PARAMS_P, P, P, P, I, P]),
     "yh_tip_detect_dev": (I, [PARAMS_P, P, P, D, P, P, P, I, P]),
     "yh_contour_dev": (I, [I, I, I, D, D, D, P, P, P, C.c_float, P, P, P, I, P]),
     "yh_contour": (I, [P, I, D, D, D, P, P, I, C.POINTER(I)]),

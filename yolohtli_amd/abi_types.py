@@ -12,6 +12,7 @@ YH_ENOTIP = 7
 YH_EOVERFLOW = 8
 YH_ENONFINITE = 9
 YH_ESTATE = 10
+YH_ESCRATCH = 11
 
 YH_MODEL_2V_REF = 0
 YH_MODEL_3V_FK = 1

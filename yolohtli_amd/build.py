@@ -25,7 +25,7 @@ HIPFLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["api.cpp", "plan.cpp", "autotune.cpp", "stepper.hip", "sr.hip", "batch.hip",
-           "adjoint.hip", "io.cpp", "general.hip", "contour.hip"]
+           "adjoint.hip", "io.cpp", "general.hip", "contour.hip", "noalloc.hip"]
 
 
 def _compile(src):

@@ -108,6 +108,7 @@ const char* yh_strerror(int code) {
     case YH_EOVERFLOW: return "tip log overflow";
     case YH_ENONFINITE: return "non-finite value in the state";
     case YH_ESTATE: return "call out of order";
+    case YH_ESCRATCH: return "grid exceeds the 2^21 cells of yh_step_noalloc's fixed scratch";
     default: return "unknown error";
   }
 }

@@ -25,18 +25,24 @@ static int64_t tile_count(int nx, int ny, int rows, int waves, int cols, int K) 
 // when there are more tiles than CUs, so two workgroups per CU overlap one's
 // loads and stores with the other's substeps (the best measured at 4096^2);
 // else 64 x 64 regions in 16 waves of 4 rows at K=8.
+yh_kernel_cfg auto_cfg(int nx, int ny, int cus) {
+  const int64_t t716 = tile_count(nx, ny, 7, 16, 1, 9);
+  const int64_t ncu = cus > 0 ? cus : 256;
+  yh_kernel_cfg c;
+  if (t716 > ncu)
+    c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 8, 8, 1};
+  else if (2 * t716 >= ncu)
+    c = yh_kernel_cfg{YH_KERNEL_TILED, 9, 7, 16, 1};
+  else
+    c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 4, 16, 1};
+  // the reflected halo is exact for at most min(nx, ny) - 1 substeps
+  c.substeps = std::max(1, std::min(c.substeps, std::min(nx, ny) - 1));
+  return c;
+}
+
 yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
   yh_kernel_cfg c = pl->cfg;
-  if (c.variant == YH_KERNEL_AUTO) {
-    const int64_t t716 = tile_count(pl->p.nx, pl->p.ny, 7, 16, 1, 9);
-    const int64_t cus = pl->cus > 0 ? pl->cus : 256;
-    if (t716 > cus)
-      c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 8, 8, 1};
-    else if (2 * t716 >= cus)
-      c = yh_kernel_cfg{YH_KERNEL_TILED, 9, 7, 16, 1};
-    else
-      c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 4, 16, 1};
-  }
+  if (c.variant == YH_KERNEL_AUTO) c = auto_cfg(pl->p.nx, pl->p.ny, pl->cus);
   if (c.variant == YH_KERNEL_TILED) {
     const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
     c.substeps = std::max(1, std::min(c.substeps, kmax));

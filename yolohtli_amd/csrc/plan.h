@@ -48,6 +48,9 @@ namespace yh {
 // Steps of this plan run on the general per-stage path (another branch, or
 // the mouse stimulus on).
 inline bool plan_general(const yh_plan* pl) { return !standard_branch(pl->p) || pl->stim_on; }
+// The YH_KERNEL_AUTO rule for an nx x ny grid on a device of `cus` compute
+// units (a tiled shape and its substeps, capped at min(nx, ny) - 1).
+yh_kernel_cfg auto_cfg(int nx, int ny, int cus);
 // Resolve YH_KERNEL_AUTO to a concrete configuration for this plan.
 yh_kernel_cfg resolve_cfg(const yh_plan* pl);
 // Advance `nsteps` on the plan's buffers, keeping u of the step before the
