@@ -127,3 +127,62 @@ def test_tip_trajectory_statistics(torch_cuda):
         print("axis", k, "mean", mg, mo, "std", sg, so)
         assert abs(mg - mo) <= 0.01 * mo
         assert abs(sg - so) <= 0.01 * so + 0.05
+
+
+def _long_run_stats():
+    import json
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, here)
+    import make_long_run_stats as S
+    with open(os.path.join(here, "long_run_stats.json")) as fh:
+        return S, json.load(fh)
+
+
+@pytest.mark.parametrize("name", ["headline", "cross_field_512"])
+def test_long_run_statistics_match_fp64(torch_cuda, name):
+    """The long-run statistics of tests/golden/long_run_stats.json (FP64
+    oracle, tests/golden/make_long_run_stats.py) from the library: the
+    headline workload (FK, 1024^2, 200 holes, random start, 50000 steps, tips
+    every 100 in the centred disk, APD at 0.15) and the 512^2 cross-field
+    spiral (10000 steps, tips over the interior).  Counts and means within
+    1%; the tip positions' mean and spread within 1% of the grid size.  Tips
+    count where they are genuine (the closed form's roots where the two
+    isolines nearly coincide depend on rounding: make_long_run_stats.genuine);
+    the raw counts are printed.  Without tips in the FP64 run the GPU must log
+    none either."""
+    S, ref = _long_run_stats()
+    want = ref[name]
+    p, mask, f, steps = S.workload(name)
+    pl = Y.Plan(p)
+    if mask is not None:
+        pl.set_mask(mask)
+    pl.set_state(*f)
+    pl.apd_enable(0.15)
+    samples = steps // S.TIP_EVERY
+    rows = []
+    fields = want["tips_raw"] > 0  # the filter reads both states at each sample
+    for k in range(samples):
+        pl.step(S.TIP_EVERY - 1)
+        past = pl.get_state()[0] if fields else None
+        pl.step(1)
+        pl.track_tips()
+        tips = pl.read_tips()
+        if len(tips):
+            on = (S.genuine(tips[:, :2], past, pl.get_state()[0], p.nx, p.ny, p.Uth) if fields
+                  else np.ones(len(tips), bool))
+            rows.extend((k, float(t[0]), float(t[1]), float(o)) for t, o in zip(tips, on))
+            pl.clear_tips()
+    a = pl.apd_get()
+    pl.close()
+    got = S.summarize(rows, a["apd1"], a["apd2"], mask, samples)
+    print(name, "GPU", got, "FP64", want)
+    if want["tips_raw"] == 0:
+        assert got["tips_raw"] == 0
+    for key in ("tips", "samples_with_tips", "apd1_cells", "apd2_cells", "apd1_mean_ms",
+                "apd2_mean_ms"):
+        assert got[key] == pytest.approx(want[key], rel=0.01, abs=1e-9), key
+    for key in ("tip_x_mean", "tip_y_mean", "tip_x_std", "tip_y_std"):
+        if key in want:
+            assert abs(got[key] - want[key]) <= 0.01 * p.nx, key
