@@ -66,6 +66,12 @@ def parse():
                          "roofline.valu_issue")
     ap.add_argument("--kernel-only", action="store_true",
                     help="only the dominant-kernel timing (for profiling); needs --cfg")
+    ap.add_argument("--batch-kernel-only", action="store_true",
+                    help="only --kernel-launches launches of the batch kernel (for profiling)")
+    ap.add_argument("--batch-cfg", default="6,8,8,1,0",
+                    help="K,rows,waves,cols,persistent of --batch-kernel-only")
+    ap.add_argument("--c0-steps", type=int, default=10000,
+                    help="steps of the 512^2 cross-field leg (0 skips it)")
     return ap.parse_args()
 
 
@@ -121,9 +127,16 @@ def copy_bandwidth(stream, mib=256, reps=20):
     return 2 * (mib << 20) / (ms / 1e3) / 1e9
 
 
-def _counter_pass(a, cfg, counter, launches=50):
-    """One rocprofv3 pass with a single counter over the --kernel-only run of
-    this configuration (the program itself follows "--"; KILL after 120 s).
+def _kernel_only_args(a, cfg, nx=None, holes=None, launches=50):
+    """Arguments of the --kernel-only run of the single-tissue stepper."""
+    return ["--kernel-only", "--cfg", ",".join(str(x) for x in cfg[1:]),
+            "--kernel-launches", str(launches), "--nx", str(nx or a.nx),
+            "--holes", str(a.holes if holes is None else holes), "--model", a.model]
+
+
+def _counter_pass(a, args, counter):
+    """One rocprofv3 pass with a single counter over a --kernel-only run of
+    bench.py with `args` (the program itself follows "--"; KILL after 120 s).
     Returns {kernel name: [values per dispatch]} or None."""
     import csv
     import glob
@@ -136,9 +149,7 @@ def _counter_pass(a, cfg, counter, launches=50):
     d = tempfile.mkdtemp(prefix="yh_traffic_", dir="/tmp")
     cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--kernel-trace",
            "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
-           os.path.abspath(__file__), "--kernel-only", "--cfg", ",".join(str(x) for x in cfg[1:]),
-           "--kernel-launches", str(launches), "--nx", str(a.nx), "--holes", str(a.holes),
-           "--model", a.model]
+           os.path.abspath(__file__)] + list(args)
     try:
         subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=150)
@@ -154,18 +165,19 @@ def _counter_pass(a, cfg, counter, launches=50):
     return out or None
 
 
-def hbm_traffic(a, cfg):
-    """HBM bytes per launch of the dominant kernel: FETCH_SIZE and WRITE_SIZE
-    (KiB per dispatch, one rocprofv3 pass each), each scaled by what the same
-    counter reads for the 256 MiB device copy the --kernel-only run makes
-    first (256 MiB read and 256 MiB written).  None if a pass fails."""
+def hbm_traffic(a, args, kernel="k_tiled"):
+    """HBM bytes per launch of a kernel: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch, one rocprofv3 pass each) over the --kernel-only run given by
+    `args`, each scaled by what the same counter reads for the 256 MiB device
+    copy that run makes first (256 MiB read and 256 MiB written).  None if a
+    pass fails."""
     total = 0.0
     parts = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        rows = _counter_pass(a, cfg, counter)
+        rows = _counter_pass(a, args, counter)
         if not rows:
             return None
-        kern = [v for k, vs in rows.items() if "k_tiled" in k for v in vs]
+        kern = [v for k, vs in rows.items() if kernel in k for v in vs]
         copy = sorted(v for k, vs in rows.items() if "copyBuffer" in k for v in vs)
         if not kern or not copy:
             return None
@@ -191,7 +203,7 @@ def valu_issue(a, cfg, t_launch_ms):
     time / launch time) is a lower bound on how busy the VALUs are.  None if
     the pass fails."""
     import torch
-    rows = _counter_pass(a, cfg, "SQ_INSTS_VALU")
+    rows = _counter_pass(a, _kernel_only_args(a, cfg), "SQ_INSTS_VALU")
     if not rows:
         return None
     kern = [v for k, vs in rows.items() if "k_tiled" in k for v in vs]
@@ -316,7 +328,45 @@ def control_grid(a, cfg):
                      "gcell_steps_per_s": round(n * n * K / (t_ms / 1e3) / 1e9, 2)}
         log(f"control {n}^2 {name}: {t_ms * 1e3:.1f} us/launch K={K}, {ach:.0f} GB/s")
     pl.close()
+    if not a.no_traffic:
+        # the state (n^2 x 25 B) exceeds the Infinity Cache: these counters
+        # see DRAM traffic, not cache hits
+        for name, c in (("tiled", cfg), ("tiled_8x8", (T.YH_KERNEL_TILED, 8, 8, 8, 1))):
+            tr = hbm_traffic(a, _kernel_only_args(a, c, nx=n, holes=holes, launches=20))
+            if tr:
+                alg = n * n * 25
+                out[name]["traffic"] = round(tr[0])
+                out[name]["traffic_over_alg"] = round(tr[0] / alg, 4)
+                out[name]["traffic_counters"] = tr[1]
+                log(f"control {n}^2 {name}: HBM traffic {tr[0] / 1e6:.1f} MB per launch "
+                    f"({tr[0] / alg:.3f}x the algorithmic {alg / 1e6:.1f} MB)")
     return out
+
+
+def batch_kernel_only(a):
+    """--batch-kernel-only: the batched workload on a fixed kernel, a 256 MiB
+    device copy (the byte counters' calibration), then --kernel-launches
+    direct launches of K steps each (no graph), for a profiler to count."""
+    import torch
+    from yolohtli_amd.batch import Batch, batched_workload
+    params, masks, fields = batched_workload(ntissues=a.batch_tissues)
+    b = Batch(params)
+    for t in range(len(params)):
+        b.set_mask(t, masks[t])
+        b.set_state(t, *fields[t])
+    K, r, w, c, pers = (int(x) for x in a.batch_cfg.split(","))
+    b.set_kernel(K, r, w, c, pers)
+    b.set_graph(False)
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    stream = torch.cuda.current_stream()
+    b.step(K * a.kernel_launches, stream)
+    torch.cuda.synchronize()
+    b.close()
 
 
 def batched(a):
@@ -361,12 +411,77 @@ def batched(a):
     rate_e = cells * n_each / (ms_e / 1e3) / 1e9
     log(f"per-tissue launches: {n_each} steps in {ms_e:.1f} ms = {rate_e:.1f} Gcell-steps/s")
     b.close()
-    return {"tissues": len(params), "cells": int(cells), "steps": a.batch_steps,
-            "gcell_steps_per_s": round(rate_b, 3), "ms": round(ms_b, 3),
-            "per_tissue_launches_gcell_steps_per_s": round(rate_e, 3),
-            "per_tissue_steps": n_each, "speedup_vs_per_tissue": round(rate_b / rate_e, 3),
-            "kernel": {"K": cfg[1], "rows": cfg[2], "waves": cfg[3], "cols": cfg[4],
-                       "persistent": pers, "graph": True}}
+    out = {"tissues": len(params), "cells": int(cells), "steps": a.batch_steps,
+           "gcell_steps_per_s": round(rate_b, 3), "ms": round(ms_b, 3),
+           "per_tissue_launches_gcell_steps_per_s": round(rate_e, 3),
+           "per_tissue_steps": n_each, "speedup_vs_per_tissue": round(rate_b / rate_e, 3),
+           "kernel": {"K": cfg[1], "rows": cfg[2], "waves": cfg[3], "cols": cfg[4],
+                      "persistent": pers, "graph": True}}
+    # per launch: every cell's three state variables read and written once
+    # plus its mask code (25 B), K steps
+    alg = int(cells) * 25
+    out["launch_us"] = round(1e3 * ms_b / (a.batch_steps / cfg[1]), 2)
+    out["alg_bytes_per_launch"] = alg
+    out["achieved_GBps"] = round(alg / (out["launch_us"] / 1e6) / 1e9, 1)
+    out["frac"] = round(out["achieved_GBps"] / HBM_PEAK_GBPS, 4)
+    if not a.no_traffic:
+        args = ["--batch-kernel-only", "--batch-tissues", str(a.batch_tissues),
+                "--batch-cfg", ",".join(str(x) for x in (cfg[1], cfg[2], cfg[3], cfg[4], pers)),
+                "--kernel-launches", "10"]
+        tr = hbm_traffic(a, args, kernel="k_batch")
+        if tr:
+            out["traffic"] = round(tr[0])
+            out["traffic_over_alg"] = round(tr[0] / alg, 4)
+            out["traffic_counters"] = tr[1]
+            log(f"batched: HBM traffic {tr[0] / 1e6:.1f} MB per launch ({tr[0] / alg:.3f}x "
+                f"the algorithmic {alg / 1e6:.1f} MB)")
+    return out
+
+
+def cross_field_512(a):
+    """The 512^2 plain configuration: FK, no holes, the cross-field start
+    (u = 1 for i < 32, v = w = 0 for j >= ny/2), --c0-steps steps on the
+    autotuned kernel, the dominant kernel's launch time by HIP events, and
+    the CPU baseline of the same workload."""
+    import torch
+    import yolohtli_amd as Y
+    from yolohtli_amd import abi_types as T
+    from yolohtli_amd import autotune
+    n = 512
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK)
+    f = Y.initGates(n, n, u_cols=32, rest=1.0, refr=0.0, nvars=3)
+    pl = Y.Plan(p)
+    pl.set_state(*f)
+    best = autotune.tune_plan(pl, f, steps=400, log=log)
+    cfg = best["cfg"]
+    pl.set_kernel(*cfg)
+    pl.set_state(*f)
+    stream = torch.cuda.current_stream()
+    pl.step(100, stream)
+    pl.set_state(*f)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    pl.step(a.c0_steps, stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    rate = n * n * a.c0_steps / (ms / 1e3) / 1e9
+    t_ms, K, alg, ach = kernel_timing(a, pl, cfg, stream, n * n, 24)
+    pl.close()
+    log(f"512^2 cross-field: {a.c0_steps} steps in {ms:.2f} ms = {rate:.1f} Gcell-steps/s; "
+        f"kernel {t_ms * 1e3:.2f} us per launch of K={K}")
+    out = {"workload": "plain stepper 512x512, FK 3V, cross-field start, no holes",
+           "steps": a.c0_steps, "ms": round(ms, 3), "gcell_steps_per_s": round(rate, 3),
+           "kernel": {"K": cfg[1], "rows": cfg[2], "waves": cfg[3], "cols": cfg[4]},
+           "launch_us": round(t_ms * 1e3, 3), "alg_bytes_per_launch": alg,
+           "achieved_GBps": round(ach, 1), "frac": round(ach / HBM_PEAK_GBPS, 4)}
+    if not a.no_cpu:
+        cpu = cpu_baseline(p, f, None, a.cpu_seconds / 2)
+        out["cpu_baseline"] = cpu
+        log(f"512^2 cross-field CPU baseline {cpu['value']:.3f} Gcell-steps/s")
+    return out
 
 
 def adjoint(a):
@@ -453,6 +568,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if a.batch_kernel_only:
+        batch_kernel_only(a)
+        return
     nx = ny = a.nx
     model = T.YH_MODEL_3V_FK if a.model == "fk" else T.YH_MODEL_2V_REF
     masked = a.holes > 0
@@ -533,7 +651,7 @@ def main():
     traffic = valu = None
     if not a.no_traffic and cfg[0] == T.YH_KERNEL_TILED:
         log("HBM traffic: FETCH_SIZE and WRITE_SIZE passes under rocprofv3 ...")
-        traffic = hbm_traffic(a, cfg)
+        traffic = hbm_traffic(a, _kernel_only_args(a, cfg))
         if traffic:
             log(f"HBM traffic {traffic[0] / 1e6:.2f} MB per launch "
                 f"({traffic[0] / alg_bytes:.3f}x the algorithmic {alg_bytes / 1e6:.2f} MB)")
@@ -550,6 +668,10 @@ def main():
     if not a.no_control and cfg[0] == T.YH_KERNEL_TILED and a.model == "fk":
         log("non-cache-resident control grid ...")
         control = control_grid(a, cfg)
+    c0 = None
+    if a.c0_steps > 0:
+        log("512^2 cross-field configuration ...")
+        c0 = cross_field_512(a)
     sr = None
     if not a.no_sr:
         log("symmetry-reduction configuration ...")
@@ -609,6 +731,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batch,
         "control_grid": control,
+        "cross_field_512": c0,
         "symmetry_reduction": sr,
         "adjoint": adj,
         "tips_logged": int(len(tips)),
