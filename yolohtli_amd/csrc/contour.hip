@@ -37,41 +37,41 @@ __global__ __launch_bounds__(256) void k_contour(int nx, int ny, int mode, float
   const int j = blockIdx.y * 4 + threadIdx.y;
   if (i >= nx || j >= ny) return;
   const int c = i + nx * j;
-  const bool sc = area ? area[c] != 0 : true;  // stimArea: avoid the pacing site
-  const float f2c = f2[c];
-  const float f2x = i < nx - 1 ? f2[c + 1] : f2c;
-  const float f2y = j < ny - 1 ? f2[c + nx] : f2c;
-  if (mode == 1) {  // space-APD contour (:55-69)
-    const float v0 = f2c, v1x = f2x, v1y = f2y;
-    const float zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
-    const float zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
-    if ((zpmx < 0.0f) || ((zpmy < 0.0f) && sc)) {  // the reference's precedence
-      const float eps = 2.220446049250313e-16f;     // MACHINE_EPS, globalVariables.cuh:26
-      const float ppx = fabsf(v0 - v1x) > eps ? (float)i + v0 / (v0 - v1x) : (float)i;
-      const float ppy = fabsf(v0 - v1y) > eps ? (float)j + v0 / (v0 - v1y) : (float)j;
-      contour_push(ppx, ppy, t, 2 * c, pts, keys, count, cap);
+  const bool in_area = area ? area[c] != 0 : true;  // stimArea: avoid the pacing site
+  const float here = f2[c];
+  const float east = i < nx - 1 ? f2[c + 1] : here;
+  const float north = j < ny - 1 ? f2[c + nx] : here;
+  // a negative product of a value with its east (north) neighbour's is a
+  // sign change across that edge; past the last column (row) the cell's
+  // own square stands in, which is never negative
+  auto crosses = [&](float a, float ae, float an, bool& xe, bool& xn) {
+    xe = (i < nx - 1 ? a * ae : a * a) < 0.0f;
+    xn = (j < ny - 1 ? a * an : a * a) < 0.0f;
+  };
+  bool xe, xn;
+  if (mode == 1) {  // zero lines of field2 (:55-69), interpolated along the edge
+    crosses(here, east, north, xe, xn);
+    if (xe || (xn && in_area)) {  // area masks the north test only, as the reference's precedence
+      const float tiny = 2.220446049250313e-16f;  // MACHINE_EPS, globalVariables.cuh:26
+      const float x = fabsf(here - east) > tiny ? (float)i + here / (here - east) : (float)i;
+      const float y = fabsf(here - north) > tiny ? (float)j + here / (here - north) : (float)j;
+      contour_push(x, y, t, 2 * c, pts, keys, count, cap);
     }
-  } else if (mode == 2) {  // single contour (:73-93)
-    if ((f1[c] < th1) && sc) {
-      const float v0 = f2c - th2, v1x = f2x - th2, v1y = f2y - th2;
-      const float zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
-      const float zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
-      if ((zpmx < 0.0f) || (zpmy < 0.0f)) contour_push((float)i, (float)j, t, 2 * c, pts, keys, count, cap);
+  } else if (mode == 2) {  // field2 = th2 where field1 < th1 (:73-93), at the cell
+    if ((f1[c] < th1) && in_area) {
+      crosses(here - th2, east - th2, north - th2, xe, xn);
+      if (xe || xn) contour_push((float)i, (float)j, t, 2 * c, pts, keys, count, cap);
     }
-  } else {  // double contour (:97-138)
-    const float V0 = f2c - th2, V1x = f2x - th2, V1y = f2y - th2;
+  } else {  // two lines around th2 inside a band of th3 (:97-138)
+    const float dh = here - th2, de = east - th2, dn = north - th2;
     const float band = th3 + 0.1f;
-    if ((f1[c] < th1) && (fabsf(V0) < band) && (fabsf(V1x) < band) && (fabsf(V1y) < band) && sc) {
-      float v0 = V0 - th2, v1x = V1x - th2, v1y = V1y - th2;
-      float zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
-      float zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
-      if ((zpmx < 0.0f) || (zpmy < 0.0f)) contour_push((float)i, (float)j, t, 2 * c, pts, keys, count, cap);
-      v0 = V0 + th3;
-      v1x = V1x + th3;
-      v1y = V1y + th3;
-      zpmx = i < nx - 1 ? v0 * v1x : v0 * v0;
-      zpmy = j < ny - 1 ? v0 * v1y : v0 * v0;
-      if ((zpmx < 0.0f) || (zpmy < 0.0f)) contour_push((float)i, (float)j, t, 2 * c + 1, pts, keys, count, cap);
+    if ((f1[c] < th1) && (fabsf(dh) < band) && (fabsf(de) < band) && (fabsf(dn) < band) &&
+        in_area) {
+      // the lower line sits at 2*th2, as the reference subtracts th2 twice
+      crosses(dh - th2, de - th2, dn - th2, xe, xn);
+      if (xe || xn) contour_push((float)i, (float)j, t, 2 * c, pts, keys, count, cap);
+      crosses(dh + th3, de + th3, dn + th3, xe, xn);
+      if (xe || xn) contour_push((float)i, (float)j, t, 2 * c + 1, pts, keys, count, cap);
     }
   }
 }
