@@ -233,12 +233,15 @@ int yh_check_finite(yh_plan* plan, void* stream);
  * While on, the plan steps on the per-stage path. */
 int yh_set_stimulus(yh_plan* plan, int on, int x, int y);
 
-/* Plan options.  YH_OPT_KEEP_PREV (default 1): every yh_step keeps u of the
+/* Plan options.  YH_OPT_SR_LAUNCHES (default 2): launches per symmetry-
+ * reduction step, 2 (the step's BFECC launch also takes the RD step and the
+ * tip test of the next step) or 3 (RD + tips in a launch of their own); the
+ * results are the same bit for bit.  YH_OPT_KEEP_PREV (default 1): every yh_step keeps u of the
  * step before its last for yh_track_tips (the last k-substep launch writes
  * it, or, for tile shapes built without that output and with APD on, a
  * single-step tail leaves it in the other state buffer); 0 drops it (no tip
  * tracking until the next yh_step with 1). */
-enum { YH_OPT_KEEP_PREV = 1 };
+enum { YH_OPT_KEEP_PREV = 1, YH_OPT_SR_LAUNCHES = 2 };
 int yh_set_option(yh_plan* plan, int option, int64_t value);
 
 /* Action-potential durations measured while stepping (sAPD_kernel,

@@ -245,7 +245,7 @@ void orc_solve_matrix(const double phi[3], const double Int[12], double cout[3])
  * examined (the reference tests solid[j-1] where it means j+1 and vice versa,
  * :47-48,54-55); a neighbour outside the domain counts as a hole, which makes
  * the masked rule equal the square-boundary rule (:112-126) on an all-tissue
- * mask. */
+ * mask, and a substitute neighbour outside the domain reads as a hole (0). */
 typedef struct orc_bf_idx {
   int W, E, S, N, iW, W2, iE, E2, iS, S2, iN, N2, sc;
 } orc_bf_idx;
@@ -266,22 +266,31 @@ static orc_bf_idx orc_bf_index(const yh_params* p, const uint8_t* solid, int i, 
     ss = j > 0;
     sn = j < ny - 1;
   }
+  /* the substitute on the other side, or -1 where that lies outside the
+   * domain (a hole on one side at the domain edge): it reads as a hole, 0.
+   * The reference takes the linear index i2d +- 1 (+- nx) there, the first
+   * cell of the next row or past the array. */
+  const int oE = i < nx - 1 ? c + 1 : -1, oW = i > 0 ? c - 1 : -1;
+  const int oN = j < ny - 1 ? c + nx : -1, oS = j > 0 ? c - nx : -1;
   orc_bf_idx r;
   r.sc = sc;
-  r.W = sc ? (sw ? c - 1 : c + 1) : c;
-  r.E = sc ? (se ? c + 1 : c - 1) : c;
-  r.S = sc ? (ss ? c - nx : c + nx) : c;
-  r.N = sc ? (sn ? c + nx : c - nx) : c;
-  r.iW = sc ? (sw ? c : c + 1) : c;
+  r.W = sc ? (sw ? c - 1 : oE) : c;
+  r.E = sc ? (se ? c + 1 : oW) : c;
+  r.S = sc ? (ss ? c - nx : oN) : c;
+  r.N = sc ? (sn ? c + nx : oS) : c;
+  r.iW = sc ? (sw ? c : oE) : c;
   r.W2 = sc ? (sw ? c - 1 : c) : c;
-  r.iE = sc ? (se ? c : c - 1) : c;
+  r.iE = sc ? (se ? c : oW) : c;
   r.E2 = sc ? (se ? c + 1 : c) : c;
-  r.iS = sc ? (ss ? c : c + nx) : c;
+  r.iS = sc ? (ss ? c : oN) : c;
   r.S2 = sc ? (ss ? c - nx : c) : c;
-  r.iN = sc ? (sn ? c : c - nx) : c;
+  r.iN = sc ? (sn ? c : oS) : c;
   r.N2 = sc ? (sn ? c + nx : c) : c;
   return r;
 }
+
+/* A field value, 0 for an index outside the domain (-1). */
+static inline double orc_bfv(const double* f, int k) { return k < 0 ? 0.0 : f[k]; }
 
 void orc_bfecc_d(const yh_params* p, const uint8_t* solid, const double* const in[3],
                  double* const out[3], const double c[3], double phit) {
@@ -317,16 +326,16 @@ void orc_bfecc_d(const yh_params* p, const uint8_t* solid, const double* const i
       for (int i = 0; i < nx; ++i) {
         const int k = i + nx * j;
         const orc_bf_idx x = orc_bf_index(p, solid, i, j);
-        const double FDx = px[k] ? g[k] - g[x.W] : g[k] - g[x.E];
-        const double FDy = py[k] ? g[k] - g[x.S] : g[k] - g[x.N];
+        const double FDx = px[k] ? g[k] - orc_bfv(g, x.W) : g[k] - orc_bfv(g, x.E);
+        const double FDy = py[k] ? g[k] - orc_bfv(g, x.S) : g[k] - orc_bfv(g, x.N);
         uf[k] = g[k] - tc * (Rx[k] * FDx + Ry[k] * FDy);
       }
     for (int j = 0; j < ny; ++j) /* phase 2: backward step + correction (:73-78) */
       for (int i = 0; i < nx; ++i) {
         const int k = i + nx * j;
         const orc_bf_idx x = orc_bf_index(p, solid, i, j);
-        const double FDx = px[k] ? uf[x.iE] - uf[x.E2] : uf[x.iW] - uf[x.W2];
-        const double FDy = py[k] ? uf[x.iN] - uf[x.N2] : uf[x.iS] - uf[x.S2];
+        const double FDx = px[k] ? orc_bfv(uf, x.iE) - uf[x.E2] : orc_bfv(uf, x.iW) - uf[x.W2];
+        const double FDy = py[k] ? orc_bfv(uf, x.iN) - uf[x.N2] : orc_bfv(uf, x.iS) - uf[x.S2];
         const double ub = uf[k] - tc * (Rx[k] * FDx + Ry[k] * FDy);
         ue[k] = g[k] - 0.5 * (ub - g[k]);
       }
@@ -334,8 +343,8 @@ void orc_bfecc_d(const yh_params* p, const uint8_t* solid, const double* const i
       for (int i = 0; i < nx; ++i) {
         const int k = i + nx * j;
         const orc_bf_idx x = orc_bf_index(p, solid, i, j);
-        const double FDx = px[k] ? ue[k] - ue[x.W] : ue[k] - ue[x.E];
-        const double FDy = py[k] ? ue[k] - ue[x.S] : ue[k] - ue[x.N];
+        const double FDx = px[k] ? ue[k] - orc_bfv(ue, x.W) : ue[k] - orc_bfv(ue, x.E);
+        const double FDy = py[k] ? ue[k] - orc_bfv(ue, x.S) : ue[k] - orc_bfv(ue, x.N);
         const double o = ue[k] - tc * (Rx[k] * FDx + Ry[k] * FDy);
         out[a][k] = x.sc ? o : 0.0;
       }

@@ -59,8 +59,23 @@ def test_host_calls_from_c(tmp_path):
     assert os.path.getsize(tmp_path / "dataparamcsv.csv") > 0
 
 
-def _plan_start(reduce_sym):
+# the paramVar fields the shim reads as REAL (float in the shim's build)
+REAL_FIELDS = ("Lx", "Ly", "dt", "diff_par", "diff_per", "rscale", "degrad", "boundaryVal",
+               "Uth", "tc", "alpha", "beta", "gamma", "delta", "eps", "mu", "theta")
+
+
+def _shim_params(reduce_sym):
+    """The parameters yh_shim_init builds from the driver's paramVar: the
+    REAL fields pass through float (the SR step uses hx, hy and dt in FP64,
+    so the rounding shows there)."""
     p = T.grid_params(N, N, reduceSym=reduce_sym)
+    for name in REAL_FIELDS:
+        setattr(p, name, float(np.float32(getattr(p, name))))
+    return p
+
+
+def _plan_start(reduce_sym):
+    p = _shim_params(reduce_sym)
     pl = Y.Plan(p)
     pl.set_state(*Y.initGates(N, N))
     return pl
@@ -92,7 +107,7 @@ def test_shim_plain_and_sr_equal_the_python_plan(torch_cuda, tmp_path):
     warm = pl.get_state()
     pl.close()
     tip0 = (float(tips[-1, 0]), float(tips[-1, 1])) if len(tips) else (N // 2, N // 2)
-    pl = Y.Plan(T.grid_params(N, N, reduceSym=1))
+    pl = Y.Plan(_shim_params(1))
     pl.set_state(*warm)
     pl.sr_begin(tip0)
     hist = pl.sr_step(200, history=True)

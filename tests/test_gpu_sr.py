@@ -105,13 +105,24 @@ def test_sr_singular_and_state_errors(torch_cuda):
         q.sr_begin((32.0, 32.0))  # plan without reduceSym
 
 
-def test_sr_masked_matches_oracle(torch_cuda):
+@pytest.mark.parametrize("edge_holes", [False, True])
+def test_sr_masked_matches_oracle(torch_cuda, edge_holes):
     """The masked branch: holes in the tissue (zero state, mirrored no-flux
     stencil, BFECC neighbour rules of advFDBFECC.cu:38-103) and the tip
-    search restricted to the centred disk (tipTracker.cu:47-51)."""
+    search restricted to the centred disk (tipTracker.cu:47-51).  With
+    edge_holes, single-cell holes next to all four domain edges, where the
+    BFECC rule's substitute neighbour lies outside the domain and reads as a
+    hole."""
     model = T.YH_MODEL_3V_FK
     st, tip0 = _spiral_state(model)
     mask = Y.make_holes(N, N, 6, 3.0, 5.0, Y.seed(9, 1, 1))
+    if edge_holes:
+        m = mask.reshape(N, N).copy()
+        m[::5, 1] = 0
+        m[2::5, N - 2] = 0
+        m[1, ::7] = 0
+        m[N - 2, 3::7] = 0
+        mask = m.reshape(-1)
     st = [np.where(mask != 0, x, 0.0).astype(np.float32) for x in st]
     p = T.grid_params(N, N, model=model, reduceSym=1, tipOffsetX=R, tipOffsetY=R, solidSwitch=1)
     pl = Y.Plan(p)
@@ -138,3 +149,51 @@ def test_sr_masked_matches_oracle(torch_cuda):
         err = np.max(np.abs(got[a] - ref.X[a]))
         print("state", a, err)
         assert err < 1e-4, (a, err)
+
+
+def _sr_run(p, st, tip0, launches, graph, mask=None, interrupt=False):
+    """100 SR steps in calls of 37 + 63 (with `interrupt`, 5 plain steps
+    between the calls: the state changed, so with two launches the next call
+    recomputes the RD output and tips of its first step); state, history and
+    tip log."""
+    pl = Y.Plan(p)
+    if mask is not None:
+        pl.set_mask(mask)
+    pl.set_state(*st)
+    pl.set_option(T.YH_OPT_SR_LAUNCHES, launches)
+    pl.sr_begin(tip0)
+    pl.sr_set_graph(graph)
+    h1 = pl.sr_step(37, history=True)
+    if interrupt:
+        pl.step(5)
+    h2 = pl.sr_step(63, history=True)
+    out = (np.concatenate([h1, h2]), pl.get_state(), pl.read_tips())
+    pl.close()
+    return out
+
+
+@pytest.mark.parametrize("model,masked", [(T.YH_MODEL_3V_FK, False), (T.YH_MODEL_3V_FK, True),
+                                          (T.YH_MODEL_2V_REF, False)])
+def test_two_launch_sr_step_equals_three_launches(torch_cuda, model, masked):
+    """The default SR step runs in two launches (the BFECC launch also takes
+    the RD step and the tip test of the next step, k_bf_rd); the three-launch
+    step (YH_OPT_SR_LAUNCHES 3) runs them in k_sr_rdtips.  Over 100 steps the
+    two agree bit for bit in state, (c, phi) history and tip log, eagerly and
+    from the graphs, also across a change of the state between calls."""
+    st, tip0 = _spiral_state(model)
+    mask = None
+    if masked:
+        rng = np.random.default_rng(4)
+        mask = (rng.random(N * N) > 0.02).astype(np.uint8)
+        st = [np.where(mask > 0, x, 0).astype(np.float32) for x in st]
+    p = T.grid_params(N, N, model=model, reduceSym=1, tipOffsetX=R, tipOffsetY=R,
+                      solidSwitch=int(masked))
+    for graph in (True, False):
+        for interrupt in (False, True):
+            a = _sr_run(p, st, tip0, 3, graph, mask, interrupt)
+            b = _sr_run(p, st, tip0, 2, graph, mask, interrupt)
+            np.testing.assert_array_equal(a[0], b[0])
+            for q in range(len(a[1])):
+                np.testing.assert_array_equal(a[1][q], b[1][q])
+            np.testing.assert_array_equal(a[2], b[2])
+            assert len(a[2]) > 0

@@ -190,3 +190,39 @@ def test_sr_step_solves_with_old_phi_then_advances_phi(model):
         assert np.isfinite(c).all() and np.abs(I).max() > 0
         np.testing.assert_array_equal(c, O.solve_matrix(phi0, I))
         np.testing.assert_array_equal(ref.phi, phi0 + dt * c)
+
+
+def test_bfecc_outside_substitute_reads_as_a_hole():
+    """Masked BFECC: a cell with a hole on one side and the domain edge on the
+    other takes its substitute neighbour outside the domain, which reads as a
+    hole.  So the domain equals the same domain grown by a ring of holes
+    (where that neighbour is an actual hole), bit for bit on the original
+    cells, to rounding.  Translation only (c_theta = 0): the advection field
+    is uniform, so the grown grid's shifted positions do not change it."""
+    rng = np.random.default_rng(11)
+    nx, ny = 23, 19
+    m = (rng.random((ny, nx)) > 0.15).astype(np.uint8)
+    for j in range(0, ny, 3):  # holes next to the east and west edges
+        m[j, 1] = 0
+        m[j, nx - 2] = 0
+    for i in range(0, nx, 4):  # and next to the south and north edges
+        m[1, i] = 0
+        m[ny - 2, i] = 0
+    f = [np.where(m > 0, rng.random((ny, nx)), 0.0) for _ in range(2)]
+    for c in ([0.7, -0.4, 0.0], [-0.5, 0.6, 0.0]):  # both upwind directions
+        p = T.grid_params(nx, ny, reduceSym=1, solidSwitch=1)
+        out = O.bfecc(p, [x.reshape(-1) for x in f], c, 0.3, solid=m.reshape(-1))
+        M = np.zeros((ny + 2, nx + 2), np.uint8)
+        M[1:-1, 1:-1] = m
+        F = [np.zeros((ny + 2, nx + 2)) for _ in range(2)]
+        for a in range(2):
+            F[a][1:-1, 1:-1] = f[a]
+        P = T.grid_params(nx + 2, ny + 2, reduceSym=1, solidSwitch=1)
+        assert O.derive(P).hx == pytest.approx(O.derive(p).hx, rel=1e-15)
+        big = O.bfecc(P, [x.reshape(-1) for x in F], c, 0.3, solid=M.reshape(-1))
+        for a in range(2):
+            got = out[a].reshape(ny, nx)
+            want = big[a].reshape(ny + 2, nx + 2)[1:-1, 1:-1]
+            # hx = Lx/(nx-1) of the grown grid differs in its last bit; a
+            # substitute read as anything but 0 differs by ~1e-2 here
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)

@@ -22,6 +22,7 @@ YH_KERNEL_DIRECT = 1
 YH_KERNEL_TILED = 2
 
 YH_OPT_KEEP_PREV = 1
+YH_OPT_SR_LAUNCHES = 2
 
 
 class YhFkParams(C.Structure):

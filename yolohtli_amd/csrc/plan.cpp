@@ -78,6 +78,7 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
   if (nsteps < 0) return YH_EINVAL;
   if (nsteps == 0) return YH_OK;
   if (pl->p.solidSwitch && !pl->mask_set) return YH_ESTATE;
+  pl->state_epoch += 1;
   const yh_kernel_cfg c = resolve_cfg(pl);
   const bool masked = pl->p.solidSwitch != 0;
   int remaining = nsteps;
@@ -244,6 +245,7 @@ int yh_set_mask(yh_plan* pl, const uint8_t* solid) {
   YH_HIP(launch_codes(pl->solid_d, pl->codes_d, pl->p.nx, pl->p.ny, nullptr));
   YH_HIP(hipDeviceSynchronize());
   pl->mask_set = true;
+  pl->state_epoch += 1;
   return YH_OK;
 }
 
@@ -256,6 +258,7 @@ int yh_set_state(yh_plan* pl, const float* u, const float* v, const float* w) {
     YH_HIP(hipMemcpy(pl->buf[pl->cur][a], src[a], bytes, hipMemcpyHostToDevice));
   pl->have_prev = false;
   pl->count = 0;
+  pl->state_epoch += 1;
   return YH_OK;
 }
 
@@ -271,6 +274,7 @@ int yh_get_state(yh_plan* pl, float* u, float* v, float* w) {
 
 int yh_state_dev(yh_plan* pl, float** u, float** v, float** w) {
   if (!pl) return YH_EINVAL;
+  pl->state_epoch += 1;  // the caller may write the state through these
   if (u) *u = pl->buf[pl->cur][0];
   if (v) *v = pl->buf[pl->cur][1];
   if (w) *w = pl->nv == 3 ? pl->buf[pl->cur][2] : nullptr;
@@ -359,6 +363,10 @@ int yh_set_option(yh_plan* pl, int option, int64_t value) {
   switch (option) {
     case YH_OPT_KEEP_PREV:
       pl->keep_prev = value != 0;
+      return YH_OK;
+    case YH_OPT_SR_LAUNCHES:
+      if (value != 2 && value != 3) return YH_EINVAL;
+      pl->sr_launches = (int)value;
       return YH_OK;
     default:
       return YH_EINVAL;

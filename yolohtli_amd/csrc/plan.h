@@ -35,6 +35,8 @@ struct yh_plan {
   yh::ApdState apd;  // device arrays (front, back, apd1, apd2, first) when apd_on
   yh_sr* sr = nullptr;
   int stim_on = 0, stim_x = 0, stim_y = 0;  // mouse stimulus (yh_set_stimulus)
+  int sr_launches = 2;      // launches per symmetry-reduction step (YH_OPT_SR_LAUNCHES)
+  long long state_epoch = 0;  // bumped whenever the state changes outside yh_sr_step
   float* gscratch[9] = {};  // stage fields of the general path (RK2/RK4), lazily allocated
   // contour search (yh_contour), lazily allocated
   float* cont_pts = nullptr;

@@ -74,6 +74,9 @@ struct SrArgs {
   int* log_keys;
   int* log_count;
   int logcap;
+  yh_tip* next_tips;  // the next step's tips (k_bf_rd)
+  int* next_keys;
+  int* next_count;
 };
 
 constexpr int kSrTipCap = 4096;
@@ -351,7 +354,9 @@ __global__ __launch_bounds__(768) void k_sr_solve_pre(SrGeom g, SrDev* st, const
 // ----------------------------------------------------------------- BFECC
 // advFDBFECC.cu:44-66 (masked) and :112-126 (square): which neighbours a cell
 // uses.  A neighbour outside the domain counts as a hole, and each cell tests
-// the mask of the neighbour it actually uses.  Bits: 0 sc (cell is tissue),
+// the mask of the neighbour it actually uses; a substitute neighbour outside
+// the domain (a hole on one side at the domain edge) reads as a hole, 0 (the
+// reference takes the linear index i2d +- 1 there, the next row's first cell).  Bits: 0 sc (cell is tissue),
 // 1 sw, 2 se, 3 ss, 4 sn (that neighbour is usable).
 template <bool MASKED>
 __device__ __forceinline__ uint32_t bf_flags(int i, int j, int nx, int ny,
@@ -405,27 +410,38 @@ __device__ __forceinline__ BfIdx bf_index(uint32_t f, int c, int stride) {
 // the domain are never read: the boundary rules substitute an inside
 // neighbour.
 constexpr int kBfTX = 64, kBfTY = 16, kBfHalo = 3, kBfThreads = 1024;
-constexpr int kBfLW = kBfTX + 2 * kBfHalo, kBfLH = kBfTY + 2 * kBfHalo;
-constexpr int kBfLN = kBfLW * kBfLH;
+// The fused step (k_bf_rd) also takes the RD step of the new state on the
+// tile grown by one cell and the tip test on the tile, which needs the new
+// state on the tile grown by two: a halo of 3 + 2.
+constexpr int kBfHaloRd = 5;
+template <int HALO>
+struct BfTile {
+  static constexpr int LW = kBfTX + 2 * HALO, LH = kBfTY + 2 * HALO, LN = LW * LH;
+};
+constexpr int kBfLW = BfTile<kBfHalo>::LW, kBfLH = BfTile<kBfHalo>::LH;
+constexpr int kBfLN = BfTile<kBfHalo>::LN;
 
-// One phase over its region (the output tile grown by 3 - PH cells), with a
-// fixed trip count so the compiler keeps several cells' LDS reads in flight.
-template <int PH, int NV>
-__device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[kBfLN],
-                                         float (*s_f)[kBfLN],
+// One phase over its region (the output tile grown by HALO - PH cells), with
+// a fixed trip count so the compiler keeps several cells' LDS reads in
+// flight.  Phase 3 writes the tile's cells to `out`; with KEEP it also keeps
+// every cell of its region (0 in holes) in s_f for what follows in the launch.
+template <int PH, int NV, int HALO = kBfHalo, bool KEEP = false>
+__device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[BfTile<HALO>::LN],
+                                         float (*s_f)[BfTile<HALO>::LN],
                                          const float* s_rx, const float* s_ry,
                                          const uint32_t* s_fl, Fields out, int i0, int j0,
                                          int nx) {
-  constexpr int W = kBfLW - 2 * PH, H = kBfLH - 2 * PH, N = W * H;
+  constexpr int LW = BfTile<HALO>::LW, LH = BfTile<HALO>::LH;
+  constexpr int W = LW - 2 * PH, H = LH - 2 * PH, N = W * H;
   constexpr int IT = (N + kBfThreads - 1) / kBfThreads;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int l = tid + it * kBfThreads;
     if (l < N) {
-      const int L = (PH + l % W) + kBfLW * (PH + l / W);
+      const int L = (PH + l % W) + LW * (PH + l / W);
       const uint32_t fl = s_fl[L];
       if (fl & 128u) {  // inside the domain
-        const BfIdx x = bf_index(fl, L, kBfLW);
+        const BfIdx x = bf_index(fl, L, LW);
         const float Rx = s_rx[L], Ry = s_ry[L];
         const bool px = fl & 32u, py = fl & 64u;
 #pragma unroll
@@ -447,8 +463,11 @@ __device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[kBfLN],
             const float FDx = px ? e[L] - e[x.W] : e[L] - e[x.E];
             const float FDy = py ? e[L] - e[x.S] : e[L] - e[x.N];
             const float o = e[L] - tc * (Rx * FDx + Ry * FDy);
-            const int gi = i0 + L % kBfLW, gj = j0 + L / kBfLW;
-            out.f[a][gi + (size_t)nx * gj] = (fl & 1u) ? o : 0.0f;
+            const float m = (fl & 1u) ? o : 0.0f;
+            const int li = L % LW, lj = L / LW;
+            if (KEEP) s_f[a][L] = m;  // uf is dead after phase 2
+            if (li >= HALO && li < HALO + kBfTX && lj >= HALO && lj < HALO + kBfTY)
+              out.f[a][(i0 + li) + (size_t)nx * (j0 + lj)] = m;
           }
         }
       }
@@ -456,20 +475,24 @@ __device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[kBfLN],
   }
 }
 
-template <bool MASKED, int NV>
-__global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CFields in, Fields out,
-                                                  const uint8_t* __restrict__ solid) {
-  __shared__ float s_q[NV][kBfLN];  // the state advected, then the corrected field ue
-  __shared__ float s_f[NV][kBfLN];  // forward step uf
-  __shared__ float s_rx[kBfLN], s_ry[kBfLN];
-  __shared__ uint32_t s_fl[kBfLN];  // bf_flags | px << 5 | py << 6 | inside << 7
+// The solve and the three BFECC phases of one step on a tile with a halo of
+// HALO cells: s_q holds the state advected, then the corrected field ue;
+// s_f the forward step uf (with KEEP, then phase 3's output); s_fl the
+// flags bf_flags | px << 5 | py << 6 | inside << 7.
+template <bool MASKED, int NV, int HALO, bool KEEP>
+__device__ __forceinline__ void bf_advect(const SrGeom& g, const SrArgs& sa, CFields in,
+                                          Fields out, const uint8_t* __restrict__ solid,
+                                          float (*s_q)[BfTile<HALO>::LN],
+                                          float (*s_f)[BfTile<HALO>::LN], float* s_rx,
+                                          float* s_ry, uint32_t* s_fl) {
+  constexpr int LW = BfTile<HALO>::LW, LN = BfTile<HALO>::LN;
   __shared__ double s_I[12];
   __shared__ double s_adv[3];  // ac2, axc, ayc
   __shared__ int s_cx, s_cy, s_found, s_base;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int nx = g.nx, ny = g.ny;
-  const int i0 = blockIdx.x * kBfTX - kBfHalo, j0 = blockIdx.y * kBfTY - kBfHalo;
+  const int i0 = blockIdx.x * kBfTX - HALO, j0 = blockIdx.y * kBfTY - HALO;
   const bool commit = blockIdx.x == 0 && blockIdx.y == 0;
   // 1 the integrals (wave w sums column w, w < 12 of the 16 waves); the state tile and the
   //   neighbour flags, which do not depend on the solve; in workgroup 0 the
@@ -478,15 +501,21 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CF
     const double v = sr_column_sum(sa.partials, sa.nparts, q);
     if (lane == 0) s_I[q] = v;
   }
-  constexpr int IT = (kBfLN + kBfThreads - 1) / kBfThreads;
+  constexpr int IT = (LN + kBfThreads - 1) / kBfThreads;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int l = tid + it * kBfThreads;
-    if (l < kBfLN) {
-      const int gi = i0 + l % kBfLW, gj = j0 + l / kBfLW;
+    if (l < LN) {
+      const int gi = i0 + l % LW, gj = j0 + l / LW;
       const bool ok = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+      // cells outside the domain hold 0 in every phase: the masked rule's
+      // substitute neighbour can lie there (a hole on one side at the domain
+      // edge), and reads as a hole
 #pragma unroll
-      for (int a = 0; a < NV; ++a) s_q[a][l] = ok ? in.f[a][gi + (size_t)nx * gj] : 0.0f;
+      for (int a = 0; a < NV; ++a) {
+        s_q[a][l] = ok ? in.f[a][gi + (size_t)nx * gj] : 0.0f;
+        s_f[a][l] = 0.0f;
+      }
       s_fl[l] = ok ? (bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u) : 0u;
     }
   }
@@ -536,11 +565,11 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CF
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int l = tid + it * kBfThreads;
-    if (l < kBfLN) {
+    if (l < LN) {
       uint32_t fl = s_fl[l];
       float rx = 0.0f, ry = 0.0f;
       if (fl & 128u) {
-        const int gi = i0 + l % kBfLW, gj = j0 + l / kBfLW;
+        const int gi = i0 + l % LW, gj = j0 + l / LW;
         double ax = g.hy * (double)gj * ac2 + axc;
         double ay = -g.hx * (double)gi * ac2 + ayc;
         if (MASKED && !(fl & 1u)) ax = ay = 0.0;  // Cxy_field_kernel :50-51
@@ -556,11 +585,93 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CF
   }
   __syncthreads();
   const float tc = (float)g.tc;
-  bf_phase<1, NV>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  bf_phase<1, NV, HALO>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
   __syncthreads();
-  bf_phase<2, NV>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  bf_phase<2, NV, HALO>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
   __syncthreads();
-  bf_phase<3, NV>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  bf_phase<3, NV, HALO, KEEP>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+}
+
+template <bool MASKED, int NV>
+__global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CFields in, Fields out,
+                                                  const uint8_t* __restrict__ solid) {
+  __shared__ float s_q[NV][kBfLN];
+  __shared__ float s_f[NV][kBfLN];
+  __shared__ float s_rx[kBfLN], s_ry[kBfLN];
+  __shared__ uint32_t s_fl[kBfLN];
+  bf_advect<MASKED, NV, kBfHalo, false>(g, sa, in, out, solid, s_q, s_f, s_rx, s_ry, s_fl);
+}
+
+// The fused step: the solve and BFECC of this step (as k_bf_fused, on a
+// halo of kBfHaloRd, keeping the new state of the tile grown by two cells in
+// LDS), then the RD step and the tip test of the next step, which
+// k_sr_rdtips would otherwise take in a launch of its own:
+//   RD of the new state on the tile grown by one, with cell_update (the
+//   arithmetic of k_sr_rdtips, bit for bit), its output of the tile to
+//   xo_out and its u to LDS;
+//   the tips of the tile between the new state (g_present) and its RD output
+//   (g_past) into the next step's tip buffer, stamped with the next count.
+template <int MODEL, bool MASKED>
+__global__ __launch_bounds__(kBfThreads) void k_bf_rd(const KParams k, SrGeom g, SrArgs sa,
+                                                       CFields in, Fields out, Fields xo_out,
+                                                       const uint8_t* __restrict__ solid,
+                                                       const uint8_t* __restrict__ codes,
+                                                       double Uth) {
+  constexpr int NV = Model<MODEL>::NV;
+  constexpr int H = kBfHaloRd, LW = BfTile<H>::LW, LN = BfTile<H>::LN;
+  __shared__ float s_q[NV][LN];
+  __shared__ float s_f[NV][LN];
+  __shared__ float s_rx[LN], s_ry[LN];
+  __shared__ uint32_t s_fl[LN];
+  bf_advect<MASKED, NV, H, true>(g, sa, in, out, solid, s_q, s_f, s_rx, s_ry, s_fl);
+  __syncthreads();
+  float* s_ro = s_q[0];  // ue is dead after phase 3: u of the RD output
+  const int tid = threadIdx.x;
+  const int nx = g.nx, ny = g.ny;
+  const int i0 = blockIdx.x * kBfTX - H, j0 = blockIdx.y * kBfTY - H;
+  const float* f[3] = {s_f[0], s_f[1], s_f[NV - 1]};
+  constexpr int RW = kBfTX + 2, RN = RW * (kBfTY + 2);
+  for (int l = tid; l < RN; l += kBfThreads) {
+    const int li = H - 1 + l % RW, lj = H - 1 + l / RW;
+    const int gi = i0 + li, gj = j0 + lj;
+    if (gi < 0 || gi >= nx || gj < 0 || gj >= ny) continue;
+    const int L = li + LW * lj;
+    const int Lw = (mirror(gi - 1, nx) - i0) + LW * lj, Le = (mirror(gi + 1, nx) - i0) + LW * lj;
+    const int Ln = li + LW * (mirror(gj + 1, ny) - j0), Ls = li + LW * (mirror(gj - 1, ny) - j0);
+    float cW = 1.f, cE = 1.f, cN = 1.f, cS = 1.f;
+    bool sc = true;
+    if (MASKED) {  // cell_update_codes
+      const uint32_t b = codes[gi + (size_t)nx * gj];
+      sc = b != kHole;
+      const uint32_t q = sc ? b : 0u;
+      cW = (float)(q & 3u);
+      cE = (float)((q >> 2) & 3u);
+      cN = (float)((q >> 4) & 3u);
+      cS = (float)((q >> 6) & 3u);
+    }
+    float xo[3], inc[3];
+    cell_update<MODEL>(k, f, L, Lw, Le, Ln, Ls, cW, cE, cN, cS, sc, xo, inc);
+    s_ro[L] = xo[0];
+    if (li >= H && li < H + kBfTX && lj >= H && lj < H + kBfTY) {
+#pragma unroll
+      for (int a = 0; a < NV; ++a) xo_out.f[a][gi + (size_t)nx * gj] = xo[a];
+    }
+  }
+  __syncthreads();
+  // physicalTime = dt*count of the next step (main.cu:885)
+  const float t = (float)(g.dt * (double)(sa.st_in->count + 1));
+  for (int l = tid; l < kBfTX * kBfTY; l += kBfThreads) {
+    const int li = H + l % kBfTX, lj = H + l / kBfTX;
+    const int gi = i0 + li, gj = j0 + lj;
+    if (gi >= nx || gj >= ny || !tip_inside(gi, gj, nx, ny, MASKED, g.r2)) continue;
+    // corners past the last column/row are the cell itself (tipTracker.cu:58-63)
+    const int L = li + LW * lj;
+    const bool ex = gi < nx - 1, ey = gj < ny - 1;
+    const int lx = ex ? L + 1 : L, ly = ey ? L + LW : L, lxy = (ex && ey) ? L + LW + 1 : L;
+    const float* pr = s_f[0];
+    tip_solve(gi, gj, nx, MASKED, Uth, pr[L], pr[lx], pr[lxy], pr[ly], s_ro[L], s_ro[lx],
+              s_ro[lxy], s_ro[ly], t, sa.next_tips, sa.next_keys, sa.next_count, kSrTipCap);
+  }
 }
 
 // ----------------------------------------------------------------- host
@@ -571,9 +682,14 @@ struct yh_sr {
   long long count = 0;      // steps taken since yh_sr_begin (the host's copy)
   double* partials = nullptr;
   int nparts = 0;
+  // tips of a step, two buffers: with two launches per step the launch that
+  // commits step n's tips (buffer n & 1) also finds step n+1's (the other)
   yh_tip* step_tips = nullptr;
   int* step_keys = nullptr;
   int* step_count = nullptr;
+  float* xo2[3] = {nullptr, nullptr, nullptr};  // RD output of odd steps (two launches)
+  bool pending = false;      // the RD output and tips of step `count` are computed ...
+  long long epoch = -1;      // ... from the state of this plan state epoch
   int reach = 0;        // largest R with R^2 < r2: the disks span (2R+1)^2 cells
   double* hist = nullptr;
   long long histcap = 0;
@@ -606,6 +722,8 @@ void sr_free(yh_sr* sr) {
   if (sr->step_tips) (void)hipFree(sr->step_tips);
   if (sr->step_keys) (void)hipFree(sr->step_keys);
   if (sr->step_count) (void)hipFree(sr->step_count);
+  for (float* q : sr->xo2)
+    if (q) (void)hipFree(q);
   if (sr->hist) (void)hipFree(sr->hist);
   delete sr;
 }
@@ -639,40 +757,57 @@ static int disk_reach(int r2) {
 
 template <int MODEL, bool MASKED>
 static void sr_launch_slices(const yh_plan* pl, const SrGeom& g, int R, CFields X, const SrDev* st,
-                             dim3 sg, dim3 sb, hipStream_t s) {
-  const yh_sr* sr = pl->sr;
+                             const yh_tip* tips, const int* keys, const int* cnt, dim3 sg,
+                             dim3 sb, hipStream_t s) {
   hipLaunchKernelGGL((k_sr_slices<MODEL, MASKED>), sg, sb, 0, s, pl->k, g, R, X, pl->codes_d, st,
-                     (const yh_tip*)sr->step_tips, (const int*)sr->step_keys,
-                     (const int*)sr->step_count, sr->partials);
+                     tips, keys, cnt, pl->sr->partials);
 }
 
+// Clears a step's tip count with a kernel (capturable, unlike a memset).
+__global__ void k_zero_count_sr(int* count) { *count = 0; }
+
 // Enqueue one SR step on stream s, reading state slot `slot` (first: the
-// count-0 step with its preliminary slices + solve):
-//   RD + tips; slices; BFECC with the solve.  Three launches in a steady step.
-static int sr_enqueue(yh_plan* pl, bool first, int slot, hipStream_t s) {
+// count-0 step with its preliminary slices + solve).
+// Three launches (YH_OPT_SR_LAUNCHES 3): RD + tips; slices; BFECC with the
+// solve.  Two (the default): slices; BFECC with the solve, then RD + tips of
+// the next step (k_bf_rd); the RD output and tips of this step come from the
+// previous step's second launch, or, with need_rd (the first step, or a
+// state changed since), from k_sr_rdtips first.  The RD output alternates
+// between buf[cur ^ 1] (even steps) and xo2 (odd steps), because a
+// workgroup's halo reads neighbours' RD output while they write the next.
+static int sr_enqueue(yh_plan* pl, bool first, bool need_rd, int slot, hipStream_t s) {
   yh_sr* sr = pl->sr;
   const SrGeom g = sr_geom(pl);
   const bool masked = pl->p.solidSwitch != 0;
+  const bool two = pl->sr_launches == 2;
   const int b = pl->cur;
   CFields X{{pl->buf[b][0], pl->buf[b][1], pl->buf[b][2]}};
-  Fields Xo{{pl->buf[b ^ 1][0], pl->buf[b ^ 1][1], pl->buf[b ^ 1][2]}};
+  Fields other{{pl->buf[b ^ 1][0], pl->buf[b ^ 1][1], pl->buf[b ^ 1][2]}};
+  Fields odd{{sr->xo2[0], sr->xo2[1], sr->xo2[2]}};
+  const Fields Xo = (two && slot == 1) ? odd : other;
+  const Fields Xo_next = slot == 1 ? other : odd;
   CFields Xoc{{Xo.f[0], Xo.f[1], Xo.f[2]}};
   Fields Xw{{pl->buf[b][0], pl->buf[b][1], pl->buf[b][2]}};
   SrDev* st_in = sr->st + slot;
   SrDev* st_out = sr->st + (slot ^ 1);
+  const int tb = two ? slot : 0;  // this step's tip buffer
+  yh_tip* tips = sr->step_tips + (size_t)tb * kSrTipCap;
+  int* keys = sr->step_keys + (size_t)tb * kSrTipCap;
+  int* cnt = sr->step_count + tb;
   const int R = sr->reach, box = 2 * R + 1;
   const int model = pl->p.model;
   // 1 RD + tips
-  {
+  if (!two || need_rd) {
+    if (two && !first) hipLaunchKernelGGL(k_zero_count_sr, dim3(1), dim3(1), 0, s, cnt);
     dim3 rb(kRtBX, kRtBY), rg((g.nx + kRtOX - 1) / kRtOX, (g.ny + kRtOY - 1) / kRtOY);
     const uint8_t* codes = pl->codes_d;
     const double Uth = pl->p.Uth;
     if (model == M2V) {
-      if (masked) hipLaunchKernelGGL((k_sr_rdtips<M2V, true>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
-      else hipLaunchKernelGGL((k_sr_rdtips<M2V, false>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
+      if (masked) hipLaunchKernelGGL((k_sr_rdtips<M2V, true>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, tips, keys, cnt);
+      else hipLaunchKernelGGL((k_sr_rdtips<M2V, false>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, tips, keys, cnt);
     } else {
-      if (masked) hipLaunchKernelGGL((k_sr_rdtips<MFK, true>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
-      else hipLaunchKernelGGL((k_sr_rdtips<MFK, false>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, sr->step_tips, sr->step_keys, sr->step_count);
+      if (masked) hipLaunchKernelGGL((k_sr_rdtips<MFK, true>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, tips, keys, cnt);
+      else hipLaunchKernelGGL((k_sr_rdtips<MFK, false>), rg, rb, 0, s, pl->k, g, X, Xo, codes, Uth, st_in, tips, keys, cnt);
     }
     YH_HIP(hipGetLastError());
   }
@@ -681,11 +816,11 @@ static int sr_enqueue(yh_plan* pl, bool first, int slot, hipStream_t s) {
   dim3 sb(kSliceBX, kSliceBY), sg((box + kSliceBX - 1) / kSliceBX, (box + kSliceBY - 1) / kSliceBY);
   for (int pass = first ? 0 : 1; pass < 2; ++pass) {
     if (model == M2V) {
-      if (masked) sr_launch_slices<M2V, true>(pl, g, R, X, st_in, sg, sb, s);
-      else sr_launch_slices<M2V, false>(pl, g, R, X, st_in, sg, sb, s);
+      if (masked) sr_launch_slices<M2V, true>(pl, g, R, X, st_in, tips, keys, cnt, sg, sb, s);
+      else sr_launch_slices<M2V, false>(pl, g, R, X, st_in, tips, keys, cnt, sg, sb, s);
     } else {
-      if (masked) sr_launch_slices<MFK, true>(pl, g, R, X, st_in, sg, sb, s);
-      else sr_launch_slices<MFK, false>(pl, g, R, X, st_in, sg, sb, s);
+      if (masked) sr_launch_slices<MFK, true>(pl, g, R, X, st_in, tips, keys, cnt, sg, sb, s);
+      else sr_launch_slices<MFK, false>(pl, g, R, X, st_in, tips, keys, cnt, sg, sb, s);
     }
     YH_HIP(hipGetLastError());
     if (pass == 0) {
@@ -694,7 +829,8 @@ static int sr_enqueue(yh_plan* pl, bool first, int slot, hipStream_t s) {
       YH_HIP(hipGetLastError());
     }
   }
-  // 3 solve + BFECC of Xo back into X, the three phases in one launch
+  // 3 solve + BFECC of Xo back into X, the three phases in one launch (with
+  //   two launches, then RD + tips of the next step)
   SrArgs sa;
   sa.st_in = st_in;
   sa.st_out = st_out;
@@ -703,15 +839,28 @@ static int sr_enqueue(yh_plan* pl, bool first, int slot, hipStream_t s) {
   sa.record = first ? 0 : 1;  // at count 0 the preliminary solve recorded
   sa.hist = sr->hist;
   sa.histcap = sr->histcap;
-  sa.step_tips = sr->step_tips;
-  sa.step_keys = sr->step_keys;
-  sa.step_count = sr->step_count;
+  sa.step_tips = tips;
+  sa.step_keys = keys;
+  sa.step_count = cnt;
   sa.log_tips = pl->tips_d;
   sa.log_keys = pl->keys_d;
   sa.log_count = pl->tipcount_d;
   sa.logcap = pl->tipcap;
+  sa.next_tips = sr->step_tips + (size_t)(tb ^ 1) * kSrTipCap;
+  sa.next_keys = sr->step_keys + (size_t)(tb ^ 1) * kSrTipCap;
+  sa.next_count = sr->step_count + (tb ^ 1);
   dim3 bg((g.nx + kBfTX - 1) / kBfTX, (g.ny + kBfTY - 1) / kBfTY);
-  if (g.nv == 3) {
+  if (two) {
+    const uint8_t* codes = pl->codes_d;
+    const double Uth = pl->p.Uth;
+    if (model == M2V) {
+      if (masked) hipLaunchKernelGGL((k_bf_rd<M2V, true>), bg, dim3(kBfThreads), 0, s, pl->k, g, sa, Xoc, Xw, Xo_next, pl->solid_d, codes, Uth);
+      else hipLaunchKernelGGL((k_bf_rd<M2V, false>), bg, dim3(kBfThreads), 0, s, pl->k, g, sa, Xoc, Xw, Xo_next, pl->solid_d, codes, Uth);
+    } else {
+      if (masked) hipLaunchKernelGGL((k_bf_rd<MFK, true>), bg, dim3(kBfThreads), 0, s, pl->k, g, sa, Xoc, Xw, Xo_next, pl->solid_d, codes, Uth);
+      else hipLaunchKernelGGL((k_bf_rd<MFK, false>), bg, dim3(kBfThreads), 0, s, pl->k, g, sa, Xoc, Xw, Xo_next, pl->solid_d, codes, Uth);
+    }
+  } else if (g.nv == 3) {
     if (masked) hipLaunchKernelGGL((k_bf_fused<true, 3>), bg, dim3(kBfThreads), 0, s, g, sa, Xoc, Xw, pl->solid_d);
     else hipLaunchKernelGGL((k_bf_fused<false, 3>), bg, dim3(kBfThreads), 0, s, g, sa, Xoc, Xw, pl->solid_d);
   } else {
@@ -747,18 +896,21 @@ int yh_sr_begin(yh_plan* pl, float tip0x, float tip0y, void* stream) {
   // each buffer on its own, so a call after a failed allocation completes it
   if (!sr->st) YH_HIP(hipMalloc(&sr->st, 2 * sizeof(SrDev)));
   if (!sr->partials) YH_HIP(hipMalloc(&sr->partials, sizeof(double) * 12 * sr->nparts));
-  if (!sr->step_tips) YH_HIP(hipMalloc(&sr->step_tips, sizeof(yh_tip) * kSrTipCap));
-  if (!sr->step_keys) YH_HIP(hipMalloc(&sr->step_keys, sizeof(int) * kSrTipCap));
-  if (!sr->step_count) YH_HIP(hipMalloc(&sr->step_count, sizeof(int)));
+  if (!sr->step_tips) YH_HIP(hipMalloc(&sr->step_tips, 2 * sizeof(yh_tip) * kSrTipCap));
+  if (!sr->step_keys) YH_HIP(hipMalloc(&sr->step_keys, 2 * sizeof(int) * kSrTipCap));
+  if (!sr->step_count) YH_HIP(hipMalloc(&sr->step_count, 2 * sizeof(int)));
+  for (int a = 0; a < pl->nv; ++a)
+    if (!sr->xo2[a]) YH_HIP(hipMalloc(&sr->xo2[a], pl->n * sizeof(float)));
   SrDev h[2];
   std::memset(h, 0, sizeof h);
   h[0].tip0x = h[1].tip0x = tip0x;
   h[0].tip0y = h[1].tip0y = tip0y;
   YH_HIP(hipMemcpyAsync(sr->st, h, sizeof h, hipMemcpyHostToDevice, s));
-  YH_HIP(hipMemsetAsync(sr->step_count, 0, sizeof(int), s));
+  YH_HIP(hipMemsetAsync(sr->step_count, 0, 2 * sizeof(int), s));
   YH_HIP(hipStreamSynchronize(s));
   sr->count = 0;
   sr->begun = true;
+  sr->pending = false;
   sr_drop_graphs(sr);
   pl->have_prev = false;
   return YH_OK;
@@ -792,16 +944,23 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
     sr_drop_graphs(sr);  // the captured steps hold the old history pointer
   }
   int rc = YH_OK;
+  const bool two = pl->sr_launches == 2;
+  // with two launches a step takes its RD output and tips from the step
+  // before; after yh_sr_begin, three-launch steps or a change of the state
+  // (plain steps, yh_set_state, yh_set_mask) they are computed first
+  bool need_rd = two && !(sr->pending && sr->epoch == pl->state_epoch);
   for (int q = 0; q < nsteps; ++q) {
     const long long cnt = count0 + q;
     const int slot = (int)(cnt & 1);
-    if (cnt == 0 || !sr->use_graph) {
-      rc = sr_enqueue(pl, cnt == 0, slot, s);
+    if (cnt == 0 || need_rd || !sr->use_graph) {
+      rc = sr_enqueue(pl, cnt == 0, need_rd || cnt == 0, slot, s);
       if (rc) return rc;
       sr->count = cnt + 1;
+      need_rd = false;
       continue;
     }
-    if (!sr->exec[slot] || sr->graph_cur[slot] != pl->cur) {
+    const int gkey = pl->cur | (two ? 2 : 0);
+    if (!sr->exec[slot] || sr->graph_cur[slot] != gkey) {
       if (sr->exec[slot]) {
         (void)hipGraphExecDestroy(sr->exec[slot]);
         (void)hipGraphDestroy(sr->graph[slot]);
@@ -811,7 +970,7 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
       if (!sr->cap_stream) YH_HIP(hipStreamCreateWithFlags(&sr->cap_stream, hipStreamNonBlocking));
       YH_HIP(hipStreamSynchronize(s));
       YH_HIP(hipStreamBeginCapture(sr->cap_stream, hipStreamCaptureModeThreadLocal));
-      rc = sr_enqueue(pl, false, slot, sr->cap_stream);
+      rc = sr_enqueue(pl, false, false, slot, sr->cap_stream);
       hipGraph_t gph = nullptr;
       hipError_t e = hipStreamEndCapture(sr->cap_stream, &gph);
       if (rc) {
@@ -821,13 +980,17 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
       YH_HIP(e);
       sr->graph[slot] = gph;
       YH_HIP(hipGraphInstantiate(&sr->exec[slot], sr->graph[slot], nullptr, nullptr, 0));
-      sr->graph_cur[slot] = pl->cur;
+      sr->graph_cur[slot] = gkey;
     }
     YH_HIP(hipGraphLaunch(sr->exec[slot], s));
     sr->count = cnt + 1;
   }
   pl->count += nsteps;
   pl->have_prev = false;
+  if (nsteps > 0) {
+    sr->pending = two;
+    sr->epoch = pl->state_epoch;
+  }
   if (c_phi_out && nsteps > 0) {
     YH_HIP(hipMemcpyAsync(c_phi_out, sr->hist + (size_t)count0 * 6, sizeof(double) * 6 * nsteps,
                           hipMemcpyDeviceToHost, s));
