@@ -80,6 +80,7 @@ struct SrArgs {
 };
 
 constexpr int kSrTipCap = 4096;
+constexpr int kSrChunk = 8;  // steady SR steps per captured graph
 constexpr int kSliceBX = 64, kSliceBY = 8;
 
 __device__ __forceinline__ bool in_disk(const SrGeom& g, int i, int j, int cx, int cy) {
@@ -695,22 +696,25 @@ struct yh_sr {
   long long histcap = 0;
   bool begun = false;
   bool use_graph = true;
-  hipGraph_t graph[2] = {nullptr, nullptr};  // one steady step per slot
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  int graph_cur[2] = {-1, -1};
+  // steady steps captured per starting slot: [slot][0] one step,
+  // [slot][1] kSrChunk steps (one graph launch for several steps)
+  hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  hipGraphExec_t exec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int graph_cur[2][2] = {{-1, -1}, {-1, -1}};
   hipStream_t cap_stream = nullptr;
 };
 
 namespace yh {
 
 static void sr_drop_graphs(yh_sr* sr) {
-  for (int q = 0; q < 2; ++q) {
-    if (sr->exec[q]) (void)hipGraphExecDestroy(sr->exec[q]);
-    if (sr->graph[q]) (void)hipGraphDestroy(sr->graph[q]);
-    sr->exec[q] = nullptr;
-    sr->graph[q] = nullptr;
-    sr->graph_cur[q] = -1;
-  }
+  for (int q = 0; q < 2; ++q)
+    for (int c = 0; c < 2; ++c) {
+      if (sr->exec[q][c]) (void)hipGraphExecDestroy(sr->exec[q][c]);
+      if (sr->graph[q][c]) (void)hipGraphDestroy(sr->graph[q][c]);
+      sr->exec[q][c] = nullptr;
+      sr->graph[q][c] = nullptr;
+      sr->graph_cur[q][c] = -1;
+    }
 }
 
 void sr_free(yh_sr* sr) {
@@ -959,18 +963,22 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
       need_rd = false;
       continue;
     }
+    // kSrChunk steps per graph launch while that many remain, else one
+    const int chunk = (nsteps - q >= kSrChunk) ? 1 : 0;
+    const int len = chunk ? kSrChunk : 1;
     const int gkey = pl->cur | (two ? 2 : 0);
-    if (!sr->exec[slot] || sr->graph_cur[slot] != gkey) {
-      if (sr->exec[slot]) {
-        (void)hipGraphExecDestroy(sr->exec[slot]);
-        (void)hipGraphDestroy(sr->graph[slot]);
-        sr->exec[slot] = nullptr;
-        sr->graph[slot] = nullptr;
+    if (!sr->exec[slot][chunk] || sr->graph_cur[slot][chunk] != gkey) {
+      if (sr->exec[slot][chunk]) {
+        (void)hipGraphExecDestroy(sr->exec[slot][chunk]);
+        (void)hipGraphDestroy(sr->graph[slot][chunk]);
+        sr->exec[slot][chunk] = nullptr;
+        sr->graph[slot][chunk] = nullptr;
       }
       if (!sr->cap_stream) YH_HIP(hipStreamCreateWithFlags(&sr->cap_stream, hipStreamNonBlocking));
       YH_HIP(hipStreamSynchronize(s));
       YH_HIP(hipStreamBeginCapture(sr->cap_stream, hipStreamCaptureModeThreadLocal));
-      rc = sr_enqueue(pl, false, false, slot, sr->cap_stream);
+      for (int k = 0; k < len && !rc; ++k)
+        rc = sr_enqueue(pl, false, false, (slot + k) & 1, sr->cap_stream);
       hipGraph_t gph = nullptr;
       hipError_t e = hipStreamEndCapture(sr->cap_stream, &gph);
       if (rc) {
@@ -978,12 +986,14 @@ int yh_sr_step(yh_plan* pl, int nsteps, void* stream, double* c_phi_out) {
         return rc;
       }
       YH_HIP(e);
-      sr->graph[slot] = gph;
-      YH_HIP(hipGraphInstantiate(&sr->exec[slot], sr->graph[slot], nullptr, nullptr, 0));
-      sr->graph_cur[slot] = gkey;
+      sr->graph[slot][chunk] = gph;
+      YH_HIP(hipGraphInstantiate(&sr->exec[slot][chunk], sr->graph[slot][chunk], nullptr, nullptr,
+                                 0));
+      sr->graph_cur[slot][chunk] = gkey;
     }
-    YH_HIP(hipGraphLaunch(sr->exec[slot], s));
-    sr->count = cnt + 1;
+    YH_HIP(hipGraphLaunch(sr->exec[slot][chunk], s));
+    sr->count = cnt + len;
+    q += len - 1;
   }
   pl->count += nsteps;
   pl->have_prev = false;
