@@ -227,8 +227,10 @@ def test_sr_frame_rebuilds_the_plain_tip_path(torch_cuda):
     print(f"steps with tips in both runs {both}/{N}; path error (cells) median "
           f"{np.median(err):.3f} p99 {np.quantile(err, 0.99):.3f} max {err.max():.3f}; tip moved "
           f"{moved:.1f} cells; phi {hist[-1, 3:]}")
+    # within 0.4 of a cell over the 2000 steps (0.27 measured); the wrong
+    # compositions of tests/test_oracle_sr_frame.py depart by over a cell
     assert both >= 0.95 * N
-    assert np.quantile(err, 0.99) <= 0.25 and err.max() <= 0.5
+    assert err.max() <= 0.4
     assert moved > 5.0 and abs(hist[-1, 5]) > 0.05
 
 

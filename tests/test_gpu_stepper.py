@@ -415,10 +415,12 @@ def test_remainder_merge_is_bitwise_equal(torch_cuda):
     equal those of eleven 9-step calls and one 1-step call, which merge
     nothing."""
     n = 512
-    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    # the masked tip search covers the centred disk (tipTracker.cu:47-51):
+    # radius 363 takes in the whole grid, where the end of the broken front
+    # of the FK cross-field start is a tip from the first samples on
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1, tipOffsetX=363,
+                      tipOffsetY=363)
     mask = Y.make_holes(n, n, 50, 3.0, 12.0, Y.seed(1, 0, 1))
-    # FK rest state (v = w = 1) with a refractory half: the broken front
-    # curls into the centred disk the masked tip search covers
     f = Y.initGates(n, n, rest=1.0, refr=0.0, nvars=3)
     out = []
     for calls in ([100], [9] * 11 + [1]):
