@@ -205,8 +205,9 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
     return hipErrorInvalidValue;
   }
   // the shapes the automatic choice takes (7 x 16 and the 8 x 16 its
-  // remainders merge into on 1024^2, 8 x 8 on grids with more tiles than CUs)
-  // also come with isotropic diffusion fixed at compile time
+  // remainders merge into on 1024^2, 8 x 8 on grids with more tiles than CUs,
+  // 4 x 16 on grids with fewer) also come with isotropic diffusion fixed at
+  // compile time
   if (k.iso && cols == 1) {
 #define YH_TILE_ISO(RR, WW)                                                                     \
   if (rows == RR && waves == WW)                                                                \
@@ -217,6 +218,7 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
     YH_TILE_ISO(7, 16)
     YH_TILE_ISO(8, 16)
     YH_TILE_ISO(8, 8)
+    YH_TILE_ISO(4, 16)
 #undef YH_TILE_ISO
   }
   if (prev) {
