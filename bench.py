@@ -589,7 +589,10 @@ def main():
 
     # ---- kernel choice (untimed): the project's autotuner, or a fixed config
     if a.cfg == "auto":
-        best = autotune.tune_plan(pl, fields, steps=400, log=log)
+        # tuned for the call size of the timed loop below (tip_every-step
+        # blocks, or one call when fewer steps are timed)
+        call = min(a.steps, a.tip_every) if a.tip_every else a.steps
+        best = autotune.tune_plan(pl, fields, steps=max(1, call), log=log)
         cfg = best["cfg"]
     else:
         K, r, w, c = (int(x) for x in a.cfg.split(","))

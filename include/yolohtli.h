@@ -269,7 +269,11 @@ int yh_contour(yh_plan* plan, int mode, double th1, double th2, double th3, cons
  * blocks, globalVariables.cuh:15-16): times the direct kernel and the tiled
  * kernel over tile shapes x substep depths on the plan's own state (restored
  * afterwards), installs the fastest, and caches it per (nx, ny, model,
- * masked) for the process.  best / best_ms (ms per step) may be NULL. */
+ * masked, call size) for the process.  steps: the number of steps per
+ * yh_step call the plan will be driven with (capped at 2000); each
+ * candidate is timed over whole calls of that size, at least 400 steps in
+ * all, so a call's remainder launch counts.  best / best_ms (ms per step)
+ * may be NULL. */
 int yh_autotune(yh_plan* plan, int steps, void* stream, yh_kernel_cfg* best, float* best_ms);
 
 /* ----------------------------------------------------- symmetry reduction */

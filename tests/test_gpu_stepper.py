@@ -477,11 +477,14 @@ def test_autotune_keeps_the_state_and_installs_its_choice(torch_cuda):
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("n,want", [(256, (8, 4, 16)), (1024, (9, 7, 16)), (4096, (8, 8, 8))])
+@pytest.mark.parametrize("n,want", [(256, (16, 4, 16)), (512, (16, 4, 16)), (540, (15, 4, 16)),
+                                    (1024, (9, 7, 16)), (4096, (8, 8, 8))])
 def test_auto_kernel_follows_the_grid(torch_cuda, n, want):
     """Without timing, AUTO takes 64 x 112 regions at K=9 when their tiles
     fill between half and all of the 256 CUs (1024^2: 253), 8 x 8 waves at
-    K=8 when there are more tiles than CUs (4096^2), else 4 x 16 at K=8."""
+    K=8 when there are more tiles than CUs (4096^2), else 4 x 16 at the
+    largest K <= 16 whose tiles number at most 256 (512^2: 16 x 16 tiles of
+    32 x 32 outputs; 540^2: K=15, 16 x 16 tiles of 34 x 34)."""
     pl = Y.Plan(T.grid_params(n, n, model=T.YH_MODEL_3V_FK))
     c = pl.get_kernel()
     pl.close()

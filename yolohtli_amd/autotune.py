@@ -9,7 +9,8 @@ from .solver import _stream_handle
 def tune_plan(plan, fields=None, steps=400, stream=None, log=None):
     """Times the candidate kernels on the plan's state and installs the fastest.
 
-    Returns {"cfg": (variant, K, rows, waves), "ms_per_step": t}."""
+    steps: the plan's call size (each candidate runs whole calls of it).
+    Returns {"cfg": (variant, K, rows, waves, cols), "ms_per_step": t}."""
     cfg = T.YhKernelCfg()
     ms = C.c_float()
     check(lib().yh_autotune(plan._h, int(steps), _stream_handle(stream), C.byref(cfg),

@@ -13,7 +13,7 @@
 
 namespace {
 
-using Key = std::tuple<int, int, int, int>;
+using Key = std::tuple<int, int, int, int, int>;
 std::mutex g_mu;
 std::map<Key, std::pair<yh_kernel_cfg, float>> g_cache;
 
@@ -54,7 +54,15 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
     return YH_OK;
   }
   hipStream_t s = (hipStream_t)stream;
-  const Key key{pl->p.nx, pl->p.ny, pl->p.model, pl->p.solidSwitch};
+  // Candidates are timed over whole calls of `steps` steps (at most 2000; a
+  // longer call's remainder weighs nothing), repeated to at least 400 steps:
+  // a call's remainder launch, or its merge into the last K-step launch,
+  // counts, so the choice fits the call size the plan will be driven with
+  // (20-step calls favour a K whose remainder merges over one a little
+  // faster per step).
+  const int call = std::min(steps, 2000);
+  const int calls = std::max(1, (400 + call - 1) / call);
+  const Key key{pl->p.nx, pl->p.ny, pl->p.model, pl->p.solidSwitch, call};
   {
     std::lock_guard<std::mutex> g(g_mu);
     auto it = g_cache.find(key);
@@ -96,12 +104,10 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
   int rc = YH_OK;
   for (const yh_kernel_cfg& c : cands) {
     pl->cfg = c;
-    const int K = c.variant == YH_KERNEL_TILED ? c.substeps : 1;
-    const int n = std::max(K, (steps / K) * K);
-    rc = plan_advance(pl, K, s);  // warm-up launch (code object load)
+    rc = plan_advance(pl, call, s);  // warm-up call (code object loads)
     if (rc) break;
     if (hipEventRecord(e0, s) != hipSuccess) { rc = YH_EHIP; break; }
-    rc = plan_advance(pl, n, s);
+    for (int i = 0; i < calls && !rc; ++i) rc = plan_advance(pl, call, s);
     if (rc) break;
     if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
       rc = YH_EHIP;
@@ -109,7 +115,7 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    const float per_step = ms / n;
+    const float per_step = ms / ((float)calls * call);
     if (per_step < bestt) {
       bestt = per_step;
       bestc = c;

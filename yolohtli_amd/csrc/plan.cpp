@@ -33,8 +33,20 @@ yh_kernel_cfg auto_cfg(int nx, int ny, int cus) {
     c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 8, 8, 1};
   else if (2 * t716 >= ncu)
     c = yh_kernel_cfg{YH_KERNEL_TILED, 9, 7, 16, 1};
-  else
-    c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 4, 16, 1};
+  else {
+    // a grid too small to fill half the CUs with 64 x 112 regions: 64 x 64
+    // regions at the deepest fusion whose tiles still fit on the CUs at one
+    // workgroup each (512^2: K=16, 256 tiles; 0.98 against 1.24 us per
+    // step at K=8), since a launch's fixed part then weighs more than the
+    // recomputed halo
+    int K = 8;
+    for (int k2 = 16; k2 > 8; --k2)
+      if (tile_count(nx, ny, 4, 16, 1, k2) <= ncu) {
+        K = k2;
+        break;
+      }
+    c = yh_kernel_cfg{YH_KERNEL_TILED, K, 4, 16, 1};
+  }
   // the reflected halo is exact for at most min(nx, ny) - 1 substeps
   c.substeps = std::max(1, std::min(c.substeps, std::min(nx, ny) - 1));
   return c;

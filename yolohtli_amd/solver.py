@@ -198,6 +198,8 @@ class Plan:
     def autotune(self, steps=400, stream=None):
         """Times the tiled shapes and depths on the current state (the state
         and step count are restored) and installs the fastest (yh_autotune).
+        steps is the call size the plan will be driven with: each candidate
+        runs whole step(steps) calls, so a call's remainder launch counts.
         Returns ((variant, K, rows, waves, cols), ms per step)."""
         cfg, ms = T.YhKernelCfg(), C.c_float()
         check(lib().yh_autotune(self._h, int(steps), _stream_handle(stream), C.byref(cfg),
