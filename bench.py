@@ -247,9 +247,9 @@ def symmetry_reduction(a):
     """Symmetry-reduction configuration: 512^2 FK, cross-field start (u=1 for
     i<32, v=w=1 for j<ny/2, 0 above), plain steps until a tip exists (the
     count is recorded), then --sr-steps co-moving-frame steps at dt=0.01 ms,
-    each three launches (RD + tips; disk centre + slices with the inner
-    products; 3x3 solve + the three BFECC phases); steady steps replayed from
-    one captured hipGraph per state slot."""
+    each two launches (disk centre + slices with the inner products; 3x3
+    solve + the three BFECC phases + the next step's RD and tip test);
+    steady steps replayed from captured hipGraphs of 8 steps."""
     import torch
     import yolohtli_amd as Y
     from yolohtli_amd import abi_types as T

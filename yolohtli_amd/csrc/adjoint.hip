@@ -370,11 +370,19 @@ __device__ __forceinline__ void adj_gtau_block(const double gt[6], double* slot)
   __shared__ double red[NT / 64][6];
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
+  // the six shuffle trees interleaved level by level, so their latencies
+  // overlap; each value's adds keep their order
+  double v[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    double s = gt[q];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) red[wv][q] = s;
+  for (int q = 0; q < 6; ++q) v[q] = gt[q];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[q] += __shfl_down(v[q], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) red[wv][q] = v[q];
   }
   __syncthreads();
   if (tid < 6) {

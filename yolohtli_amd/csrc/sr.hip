@@ -1,6 +1,6 @@
 // sr.hip -- symmetry-reduction (co-moving frame) step on the device.
 //
-// One SR step follows display() (main.cu:894-953) in three launches:
+// One SR step follows display() (main.cu:894-953).  Three kernels make it:
 //   1 k_sr_rdtips   RD X -> Xo (reactionDiffusion_kernel) and the tips
 //                   between Xo.u and X.u (tip_wrapper, main.cu:900); blocks
 //                   overlap by one column and row, so every corner of an
@@ -18,14 +18,20 @@
 //                   (advFDBFECC.cu:18-165), the three dependent phases through
 //                   an LDS tile with a 3-cell halo.  Workgroup 0 commits the
 //                   step: c, phi += dt*c, centre, count, the tips to the log.
+// By default a steady step is two launches: k_sr_slices, then k_bf_rd, which
+// is k_bf_fused on a 5-cell halo followed by the next step's RD and tip test
+// (what k_sr_rdtips would compute, bit for bit) on the same tile; the first
+// step of a run, or of one after the state changed from outside, starts with
+// k_sr_rdtips.  YH_OPT_SR_LAUNCHES 3 keeps the three separate launches.
 // The SR state has two slots: a step reads slot count&1 and workgroup 0 of
-// launch 3 writes slot (count+1)&1, so no workgroup reads what another writes
-// in the same launch.  At count 0 a preliminary slices + solve pass runs
-// first (main.cu:910-921).  Nothing leaves the device; a steady step is
-// captured once per slot into a hipGraph and replayed.  Results cross
-// workgroups only at kernel boundaries: on this 8-XCD part a device-scope
-// fence writes back the whole L2 of its XCD, so a last-workgroup election in
-// every workgroup costs far more than a launch.
+// the BFECC launch writes slot (count+1)&1, so no workgroup reads what
+// another writes in the same launch.  At count 0 a preliminary slices + solve
+// pass runs first (main.cu:910-921).  Nothing leaves the device; steady
+// steps are captured into hipGraphs (8 steps to a graph where at least 8
+// remain) and replayed.  Results cross workgroups only at kernel
+// boundaries: on this 8-XCD part a device-scope fence writes back the whole
+// L2 of its XCD, so a last-workgroup election in every workgroup costs far
+// more than a launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -145,32 +151,50 @@ __global__ __launch_bounds__(kRtBX* kRtBY) void k_sr_rdtips(const KParams k, con
 // so the wave maximum names one slot.
 __device__ __forceinline__ void sr_centre(const SrGeom& g, const SrDev* st, const yh_tip* tips,
                                           const int* keys, const int* count, int& cx, int& cy) {
-  if (st->count == 0) {
+  const int lane = __lane_id();
+  // the step count, the tip count, the first 64 keys and their tips are read
+  // together (slots past the tip count are ignored), so the centre costs one
+  // memory round trip when a step has at most 64 tips
+  const long long step = st->count;
+  const int k0 = keys[lane];
+  const yh_tip t0 = tips[lane];
+  const int cnt = *count;
+  if (step == 0) {
     cx = __float2int_rn(st->tip0x - (float)(g.nx / 2));
     cy = __float2int_rn(st->tip0y - (float)(g.ny / 2));
     return;
   }
-  const int lane = __lane_id();
-  const int n = min(*count, kSrTipCap);
+  const int n = min(cnt, kSrTipCap);
   int bk = -1, bs = -1;
-  for (int q = lane; q < n; q += 64) {
+  float bx = 0.0f, by = 0.0f;
+  if (lane < n) {
+    bk = k0;
+    bs = lane;
+    bx = t0.x;
+    by = t0.y;
+  }
+  for (int q = lane + 64; q < n; q += 64) {
     const int kq = keys[q];
     if (kq > bk) {
       bk = kq;
       bs = q;
+      bx = tips[q].x;
+      by = tips[q].y;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
     const int ok = __shfl_xor(bk, off, 64), os = __shfl_xor(bs, off, 64);
+    const float ox = __shfl_xor(bx, off, 64), oy = __shfl_xor(by, off, 64);
     if (ok > bk) {
       bk = ok;
       bs = os;
+      bx = ox;
+      by = oy;
     }
   }
   if (bs >= 0) {
-    const yh_tip tp = tips[bs];
-    cx = __float2int_rn(tp.x - (float)(g.nx / 2));
-    cy = __float2int_rn(tp.y - (float)(g.ny / 2));
+    cx = __float2int_rn(bx - (float)(g.nx / 2));
+    cy = __float2int_rn(by - (float)(g.ny / 2));
   } else {
     cx = st->cx;
     cy = st->cy;
@@ -259,11 +283,16 @@ __global__ __launch_bounds__(kSliceBX* kSliceBY) void k_sr_slices(
   __shared__ double red[kSliceBX * kSliceBY / 64][12];
   const int tid = threadIdx.y * kSliceBX + threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
+  // the 12 shuffle trees interleaved level by level (one basic block, so
+  // their latencies overlap); each value's adds keep their order
 #pragma unroll
-  for (int q = 0; q < 12; ++q) {
-    double v = acc[q];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) red[wv][q] = v;
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 12; ++q) acc[q] += __shfl_down(acc[q], off, 64);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 12; ++q) red[wv][q] = acc[q];
   }
   __syncthreads();
   if (tid < 12) {
@@ -281,7 +310,19 @@ __device__ __forceinline__ double sr_column_sum(const double* partials, int npar
   const int lane = __lane_id();
   double s = 0.0;
   const double* col = partials + (size_t)q * nparts;
-  for (int b = lane; b < nparts; b += 64) s += col[b];
+  // four loads in flight per lane before the adds, which keep the order
+  // lane, lane + 64, lane + 128, ... of a one-at-a-time loop
+  for (int b0 = 0; b0 < nparts; b0 += 256) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int b = b0 + lane + 64 * u;
+      v[u] = b < nparts ? col[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (b0 + lane + 64 * u < nparts) s += v[u];
+  }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   return s;
 }
@@ -291,14 +332,13 @@ __device__ __forceinline__ double sr_column_sum(const double* partials, int npar
 // (:45-56) with the new c and the current phi, and, when advance, the Euler
 // update phi += dt*c (reduceSymStart stays set, main.cu:935-941) and the
 // step count.
-__device__ void sr_solve(const SrGeom& g, const double* sums, SrDev& o, bool advance) {
+__device__ void sr_solve(const SrGeom& g, const double* sums, SrDev& o, bool advance, double sp,
+                         double cp) {
   double I[12];
   for (int p = 0; p < 12; ++p) {
     I[p] = sums[p] * (g.hx * g.hy);
     o.I[p] = I[p];
   }
-  double sp, cp;
-  sincos(o.phi[2], &sp, &cp);
   const double a1 = I[0] * cp + I[1] * sp, a2 = I[1] * cp - I[0] * sp, a3 = I[2];
   const double b1 = I[3] * cp + I[4] * sp, b2 = I[4] * cp - I[3] * sp, b3 = I[5];
   const double C1 = I[6] * cp + I[7] * sp, C2 = I[7] * cp - I[6] * sp, C3 = I[8];
@@ -348,7 +388,9 @@ __global__ __launch_bounds__(768) void k_sr_solve_pre(SrGeom g, SrDev* st, const
   if (threadIdx.x != 0) return;
   SrDev o = *st;
   sr_record(o, hist, histcap);
-  sr_solve(g, Is, o, false);
+  double sp, cp;
+  sincos(o.phi[2], &sp, &cp);
+  sr_solve(g, Is, o, false, sp, cp);
   *st = o;
 }
 
@@ -480,8 +522,9 @@ __device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[BfTile<
 // HALO cells: s_q holds the state advected, then the corrected field ue;
 // s_f the forward step uf (with KEEP, then phase 3's output); s_fl the
 // flags bf_flags | px << 5 | py << 6 | inside << 7.
+// Returns the step count of the state it advanced (from LDS).
 template <bool MASKED, int NV, int HALO, bool KEEP>
-__device__ __forceinline__ void bf_advect(const SrGeom& g, const SrArgs& sa, CFields in,
+__device__ __forceinline__ long long bf_advect(const SrGeom& g, const SrArgs& sa, CFields in,
                                           Fields out, const uint8_t* __restrict__ solid,
                                           float (*s_q)[BfTile<HALO>::LN],
                                           float (*s_f)[BfTile<HALO>::LN], float* s_rx,
@@ -490,6 +533,10 @@ __device__ __forceinline__ void bf_advect(const SrGeom& g, const SrArgs& sa, CFi
   __shared__ double s_I[12];
   __shared__ double s_adv[3];  // ac2, axc, ayc
   __shared__ int s_cx, s_cy, s_found, s_base;
+  __shared__ SrDev s_st;      // this step's SR state
+  __shared__ double s_sc[2];  // sine and cosine of its angle phi[2]
+  static_assert(sizeof(SrDev) % 8 == 0 && sizeof(SrDev) / 8 <= 64, "SrDev copy by one wave");
+  static_assert(kBfThreads / 64 >= 14, "waves 12 and 13 fetch the state");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
   const int nx = g.nx, ny = g.ny;
@@ -501,6 +548,18 @@ __device__ __forceinline__ void bf_advect(const SrGeom& g, const SrArgs& sa, CFi
   for (int q = wv; q < 12; q += kBfThreads / 64) {
     const double v = sr_column_sum(sa.partials, sa.nparts, q);
     if (lane == 0) s_I[q] = v;
+  }
+  // the state of this step and the sine and cosine of its angle, which do
+  // not depend on the integrals, by two of the waves that sum no column
+  if (wv == 12) {
+    if (lane < (int)(sizeof(SrDev) / 8))
+      reinterpret_cast<unsigned long long*>(&s_st)[lane] =
+          reinterpret_cast<const unsigned long long*>(sa.st_in)[lane];
+  } else if (wv == 13 && lane == 0) {
+    double sp, cp;
+    sincos(sa.st_in->phi[2], &sp, &cp);
+    s_sc[0] = sp;
+    s_sc[1] = cp;
   }
   constexpr int IT = (LN + kBfThreads - 1) / kBfThreads;
 #pragma unroll
@@ -535,9 +594,9 @@ __device__ __forceinline__ void bf_advect(const SrGeom& g, const SrArgs& sa, CFi
   // 2 the solve, in every workgroup from the same sums; workgroup 0 writes
   //   the next state slot and moves this step's tips to the log
   if (tid == 0) {
-    SrDev o = *sa.st_in;
+    SrDev o = s_st;
     if (commit && sa.record) sr_record(o, sa.hist, sa.histcap);
-    sr_solve(g, s_I, o, true);
+    sr_solve(g, s_I, o, true, s_sc[0], s_sc[1]);
     s_adv[0] = o.ac2;
     s_adv[1] = o.axc;
     s_adv[2] = o.ayc;
@@ -591,6 +650,7 @@ __device__ __forceinline__ void bf_advect(const SrGeom& g, const SrArgs& sa, CFi
   bf_phase<2, NV, HALO>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
   __syncthreads();
   bf_phase<3, NV, HALO, KEEP>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  return s_st.count;
 }
 
 template <bool MASKED, int NV>
@@ -624,7 +684,8 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_rd(const KParams k, SrGeom g,
   __shared__ float s_f[NV][LN];
   __shared__ float s_rx[LN], s_ry[LN];
   __shared__ uint32_t s_fl[LN];
-  bf_advect<MASKED, NV, H, true>(g, sa, in, out, solid, s_q, s_f, s_rx, s_ry, s_fl);
+  const long long count =
+      bf_advect<MASKED, NV, H, true>(g, sa, in, out, solid, s_q, s_f, s_rx, s_ry, s_fl);
   __syncthreads();
   float* s_ro = s_q[0];  // ue is dead after phase 3: u of the RD output
   const int tid = threadIdx.x;
@@ -660,7 +721,7 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_rd(const KParams k, SrGeom g,
   }
   __syncthreads();
   // physicalTime = dt*count of the next step (main.cu:885)
-  const float t = (float)(g.dt * (double)(sa.st_in->count + 1));
+  const float t = (float)(g.dt * (double)(count + 1));
   for (int l = tid; l < kBfTX * kBfTY; l += kBfThreads) {
     const int li = H + l % kBfTX, lj = H + l / kBfTX;
     const int gi = i0 + li, gj = j0 + lj;
