@@ -283,16 +283,32 @@ __global__ __launch_bounds__(kSliceBX* kSliceBY) void k_sr_slices(
   __shared__ double red[kSliceBX * kSliceBY / 64][12];
   const int tid = threadIdx.y * kSliceBX + threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6;
-  // the 12 shuffle trees interleaved level by level (one basic block, so
-  // their latencies overlap); each value's adds keep their order
+  // Sums over the wave's 64 lanes with the pairs of a shfl_down tree (at
+  // each level lane i's partial meets lane i + off's, so every sum equals
+  // that tree's bit for bit: IEEE addition commutes) but fewer exchanges:
+  // at the first level the lower half of the wave keeps values 0-5 and the
+  // upper half 6-11, at the second each quarter keeps three, so 6 + 3 +
+  // 4 x 3 exchanges instead of 6 x 12.  Values 3r..3r+2 end in lane 16r.
+  const bool up32 = (lane & 32) != 0, up16 = (lane & 16) != 0;
+  double h[6], t[3];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int q = 0; q < 12; ++q) acc[q] += __shfl_down(acc[q], off, 64);
+  for (int p = 0; p < 6; ++p) {
+    const double give = up32 ? acc[p] : acc[6 + p];
+    h[p] = (up32 ? acc[6 + p] : acc[p]) + __shfl_xor(give, 32, 64);
   }
-  if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < 12; ++q) red[wv][q] = acc[q];
+  for (int p = 0; p < 3; ++p) {
+    const double give = up16 ? h[p] : h[3 + p];
+    t[p] = (up16 ? h[3 + p] : h[p]) + __shfl_xor(give, 16, 64);
+  }
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) t[p] += __shfl_xor(t[p], off, 64);
+  }
+  if ((lane & 15) == 0) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p) red[wv][3 * (lane >> 4) + p] = t[p];
   }
   __syncthreads();
   if (tid < 12) {
