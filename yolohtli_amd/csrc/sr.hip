@@ -480,54 +480,76 @@ struct BfTile {
 constexpr int kBfLW = BfTile<kBfHalo>::LW, kBfLH = BfTile<kBfHalo>::LH;
 constexpr int kBfLN = BfTile<kBfHalo>::LN;
 
-// One phase over its region (the output tile grown by HALO - PH cells), with
-// a fixed trip count so the compiler keeps several cells' LDS reads in
-// flight.  Phase 3 writes the tile's cells to `out`; with KEEP it also keeps
-// every cell of its region (0 in holes) in s_f for what follows in the launch.
+// Cells of a thread in the three phases: the phase-1 region (the output
+// tile grown by HALO - 1 cells), cell l = tid + it * kBfThreads of it, so a
+// thread keeps its cells' flags, Courant factors and own values in
+// registers from phase to phase and reads only neighbours from LDS.
+template <int HALO>
+struct BfCells {
+  static constexpr int W1 = BfTile<HALO>::LW - 2, H1 = BfTile<HALO>::LH - 2, N1 = W1 * H1;
+  static constexpr int IT = (N1 + kBfThreads - 1) / kBfThreads;
+};
+
+// Per-thread state of its cells: flags (bit 7: inside the domain, bits 5-6:
+// the advection's signs), |c| dt/h per axis, and per variable the input q,
+// the forward step uf and the corrected field ue of the cell itself.
+template <int NV, int HALO>
+struct BfOwn {
+  static constexpr int IT = BfCells<HALO>::IT;
+  uint32_t fl[IT];
+  float rx[IT], ry[IT];
+  float q[IT][NV], f[IT][NV], e[IT][NV];
+};
+
+// One phase over its region (the output tile grown by HALO - PH cells).
+// Phase 3 writes the tile's cells to `out`; with KEEP it also keeps every
+// cell of its region (0 in holes) in s_f for what follows in the launch.
 template <int PH, int NV, int HALO = kBfHalo, bool KEEP = false>
 __device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[BfTile<HALO>::LN],
-                                         float (*s_f)[BfTile<HALO>::LN],
-                                         const float* s_rx, const float* s_ry,
-                                         const uint32_t* s_fl, Fields out, int i0, int j0,
-                                         int nx) {
+                                         float (*s_f)[BfTile<HALO>::LN], BfOwn<NV, HALO>& o,
+                                         Fields out, int i0, int j0, int nx) {
   constexpr int LW = BfTile<HALO>::LW, LH = BfTile<HALO>::LH;
-  constexpr int W = LW - 2 * PH, H = LH - 2 * PH, N = W * H;
-  constexpr int IT = (N + kBfThreads - 1) / kBfThreads;
+  constexpr int W1 = BfCells<HALO>::W1, N1 = BfCells<HALO>::N1, IT = BfCells<HALO>::IT;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int l = tid + it * kBfThreads;
-    if (l < N) {
-      const int L = (PH + l % W) + LW * (PH + l / W);
-      const uint32_t fl = s_fl[L];
-      if (fl & 128u) {  // inside the domain
-        const BfIdx x = bf_index(fl, L, LW);
-        const float Rx = s_rx[L], Ry = s_ry[L];
-        const bool px = fl & 32u, py = fl & 64u;
+    const int li = 1 + l % W1, lj = 1 + l / W1;
+    const bool in_region = l < N1 && li >= PH && li < LW - PH && lj >= PH && lj < LH - PH;
+    const uint32_t fl = o.fl[it];
+    if (in_region && (fl & 128u)) {  // inside the domain
+      const int L = li + LW * lj;
+      const BfIdx x = bf_index(fl, L, LW);
+      const float Rx = o.rx[it], Ry = o.ry[it];
+      const bool px = fl & 32u, py = fl & 64u;
 #pragma unroll
-        for (int a = 0; a < NV; ++a) {
-          if (PH == 1) {  // forward step (advFDBFECC.cu:128-131)
-            const float* q = s_q[a];
-            const float FDx = px ? q[L] - q[x.W] : q[L] - q[x.E];
-            const float FDy = py ? q[L] - q[x.S] : q[L] - q[x.N];
-            s_f[a][L] = q[L] - tc * (Rx * FDx + Ry * FDy);
-          } else if (PH == 2) {  // backward step and error correction (:134-139)
-            const float* f = s_f[a];
-            const float FDx = px ? f[x.iE] - f[x.E2] : f[x.iW] - f[x.W2];
-            const float FDy = py ? f[x.iN] - f[x.N2] : f[x.iS] - f[x.S2];
-            const float ub = f[L] - tc * (Rx * FDx + Ry * FDy);
-            const float g0 = s_q[a][L];
-            s_q[a][L] = g0 - 0.5f * (ub - g0);  // ue over the input: only cell L reads it
-          } else {  // forward step from the corrected field (:141-144)
-            const float* e = s_q[a];
-            const float FDx = px ? e[L] - e[x.W] : e[L] - e[x.E];
-            const float FDy = py ? e[L] - e[x.S] : e[L] - e[x.N];
-            const float o = e[L] - tc * (Rx * FDx + Ry * FDy);
-            const float m = (fl & 1u) ? o : 0.0f;
-            const int li = L % LW, lj = L / LW;
-            if (KEEP) s_f[a][L] = m;  // uf is dead after phase 2
-            if (li >= HALO && li < HALO + kBfTX && lj >= HALO && lj < HALO + kBfTY)
-              out.f[a][(i0 + li) + (size_t)nx * (j0 + lj)] = m;
-          }
+      for (int a = 0; a < NV; ++a) {
+        if (PH == 1) {  // forward step (advFDBFECC.cu:128-131)
+          const float* q = s_q[a];
+          const float qL = o.q[it][a];
+          const float FDx = px ? qL - q[x.W] : qL - q[x.E];
+          const float FDy = py ? qL - q[x.S] : qL - q[x.N];
+          const float uf = qL - tc * (Rx * FDx + Ry * FDy);
+          s_f[a][L] = uf;
+          o.f[it][a] = uf;
+        } else if (PH == 2) {  // backward step and error correction (:134-139)
+          const float* f = s_f[a];
+          const float FDx = px ? f[x.iE] - f[x.E2] : f[x.iW] - f[x.W2];
+          const float FDy = py ? f[x.iN] - f[x.N2] : f[x.iS] - f[x.S2];
+          const float ub = o.f[it][a] - tc * (Rx * FDx + Ry * FDy);
+          const float g0 = o.q[it][a];
+          const float ue = g0 - 0.5f * (ub - g0);
+          s_q[a][L] = ue;  // over the input: only cell L read it there
+          o.e[it][a] = ue;
+        } else {  // forward step from the corrected field (:141-144)
+          const float* e = s_q[a];
+          const float eL = o.e[it][a];
+          const float FDx = px ? eL - e[x.W] : eL - e[x.E];
+          const float FDy = py ? eL - e[x.S] : eL - e[x.N];
+          const float r = eL - tc * (Rx * FDx + Ry * FDy);
+          const float m = (fl & 1u) ? r : 0.0f;
+          if (KEEP) s_f[a][L] = m;  // uf is dead after phase 2
+          if (li >= HALO && li < HALO + kBfTX && lj >= HALO && lj < HALO + kBfTY)
+            out.f[a][(i0 + li) + (size_t)nx * (j0 + lj)] = m;
         }
       }
     }
@@ -536,16 +558,17 @@ __device__ __forceinline__ void bf_phase(int tid, float tc, float (*s_q)[BfTile<
 
 // The solve and the three BFECC phases of one step on a tile with a halo of
 // HALO cells: s_q holds the state advected, then the corrected field ue;
-// s_f the forward step uf (with KEEP, then phase 3's output); s_fl the
-// flags bf_flags | px << 5 | py << 6 | inside << 7.
+// s_f the forward step uf (with KEEP, then phase 3's output).  A thread's
+// cells carry their flags bf_flags | px << 5 | py << 6 | inside << 7 in
+// registers (BfOwn).
 // Returns the step count of the state it advanced (from LDS).
 template <bool MASKED, int NV, int HALO, bool KEEP>
 __device__ __forceinline__ long long bf_advect(const SrGeom& g, const SrArgs& sa, CFields in,
                                           Fields out, const uint8_t* __restrict__ solid,
                                           float (*s_q)[BfTile<HALO>::LN],
-                                          float (*s_f)[BfTile<HALO>::LN], float* s_rx,
-                                          float* s_ry, uint32_t* s_fl) {
+                                          float (*s_f)[BfTile<HALO>::LN]) {
   constexpr int LW = BfTile<HALO>::LW, LN = BfTile<HALO>::LN;
+  constexpr int W1 = BfCells<HALO>::W1, N1 = BfCells<HALO>::N1, IC = BfCells<HALO>::IT;
   __shared__ double s_I[12];
   __shared__ double s_adv[3];  // ac2, axc, ayc
   __shared__ int s_cx, s_cy, s_found, s_base;
@@ -592,8 +615,16 @@ __device__ __forceinline__ long long bf_advect(const SrGeom& g, const SrArgs& sa
         s_q[a][l] = ok ? in.f[a][gi + (size_t)nx * gj] : 0.0f;
         s_f[a][l] = 0.0f;
       }
-      s_fl[l] = ok ? (bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u) : 0u;
     }
+  }
+  // the flags of this thread's cells (the phase-1 region)
+  BfOwn<NV, HALO> o;
+#pragma unroll
+  for (int it = 0; it < IC; ++it) {
+    const int l = tid + it * kBfThreads;
+    const int gi = i0 + 1 + l % W1, gj = j0 + 1 + l / W1;
+    const bool ok = l < N1 && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+    o.fl[it] = ok ? (bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u) : 0u;
   }
   if (commit && wv == 0) {
     int cx, cy;
@@ -635,37 +666,39 @@ __device__ __forceinline__ long long bf_advect(const SrGeom& g, const SrArgs& sa
   }
   __syncthreads();
   if (commit && tid == 0) *sa.step_count = 0;
-  // 3 advection coefficients of each tile cell (Cxy_field_kernel,
-  //   symmetryReduction.cu:45-56, negated as at advFDBFECC.cu:30-31)
+  // 3 advection coefficients of this thread's cells (Cxy_field_kernel,
+  //   symmetryReduction.cu:45-56, negated as at advFDBFECC.cu:30-31) and
+  //   their input values; phase 1 reads nothing else another thread wrote
+  //   after the barrier above
   const double ac2 = s_adv[0], axc = s_adv[1], ayc = s_adv[2];
 #pragma unroll
-  for (int it = 0; it < IT; ++it) {
+  for (int it = 0; it < IC; ++it) {
     const int l = tid + it * kBfThreads;
-    if (l < LN) {
-      uint32_t fl = s_fl[l];
-      float rx = 0.0f, ry = 0.0f;
-      if (fl & 128u) {
-        const int gi = i0 + l % LW, gj = j0 + l / LW;
-        double ax = g.hy * (double)gj * ac2 + axc;
-        double ay = -g.hx * (double)gi * ac2 + ayc;
-        if (MASKED && !(fl & 1u)) ax = ay = 0.0;  // Cxy_field_kernel :50-51
-        const double cx = -ax, cy = -ay;          // advFDBFECC.cu:30-31
-        rx = (float)(fabs(cx) * g.dtx);
-        ry = (float)(fabs(cy) * g.dty);
-        fl |= (cx > 0.0 ? 32u : 0u) | (cy > 0.0 ? 64u : 0u);
-      }
-      s_rx[l] = rx;
-      s_ry[l] = ry;
-      s_fl[l] = fl;
+    const int li = 1 + l % W1, lj = 1 + l / W1;
+    uint32_t fl = o.fl[it];
+    float rx = 0.0f, ry = 0.0f;
+    if (fl & 128u) {
+      const int gi = i0 + li, gj = j0 + lj;
+      double ax = g.hy * (double)gj * ac2 + axc;
+      double ay = -g.hx * (double)gi * ac2 + ayc;
+      if (MASKED && !(fl & 1u)) ax = ay = 0.0;  // Cxy_field_kernel :50-51
+      const double cx = -ax, cy = -ay;          // advFDBFECC.cu:30-31
+      rx = (float)(fabs(cx) * g.dtx);
+      ry = (float)(fabs(cy) * g.dty);
+      fl |= (cx > 0.0 ? 32u : 0u) | (cy > 0.0 ? 64u : 0u);
+#pragma unroll
+      for (int a = 0; a < NV; ++a) o.q[it][a] = s_q[a][li + LW * lj];
     }
+    o.rx[it] = rx;
+    o.ry[it] = ry;
+    o.fl[it] = fl;
   }
-  __syncthreads();
   const float tc = (float)g.tc;
-  bf_phase<1, NV, HALO>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  bf_phase<1, NV, HALO>(tid, tc, s_q, s_f, o, out, i0, j0, nx);
   __syncthreads();
-  bf_phase<2, NV, HALO>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  bf_phase<2, NV, HALO>(tid, tc, s_q, s_f, o, out, i0, j0, nx);
   __syncthreads();
-  bf_phase<3, NV, HALO, KEEP>(tid, tc, s_q, s_f, s_rx, s_ry, s_fl, out, i0, j0, nx);
+  bf_phase<3, NV, HALO, KEEP>(tid, tc, s_q, s_f, o, out, i0, j0, nx);
   return s_st.count;
 }
 
@@ -674,9 +707,7 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_fused(SrGeom g, SrArgs sa, CF
                                                   const uint8_t* __restrict__ solid) {
   __shared__ float s_q[NV][kBfLN];
   __shared__ float s_f[NV][kBfLN];
-  __shared__ float s_rx[kBfLN], s_ry[kBfLN];
-  __shared__ uint32_t s_fl[kBfLN];
-  bf_advect<MASKED, NV, kBfHalo, false>(g, sa, in, out, solid, s_q, s_f, s_rx, s_ry, s_fl);
+  bf_advect<MASKED, NV, kBfHalo, false>(g, sa, in, out, solid, s_q, s_f);
 }
 
 // The fused step: the solve and BFECC of this step (as k_bf_fused, on a
@@ -698,10 +729,8 @@ __global__ __launch_bounds__(kBfThreads) void k_bf_rd(const KParams k, SrGeom g,
   constexpr int H = kBfHaloRd, LW = BfTile<H>::LW, LN = BfTile<H>::LN;
   __shared__ float s_q[NV][LN];
   __shared__ float s_f[NV][LN];
-  __shared__ float s_rx[LN], s_ry[LN];
-  __shared__ uint32_t s_fl[LN];
   const long long count =
-      bf_advect<MASKED, NV, H, true>(g, sa, in, out, solid, s_q, s_f, s_rx, s_ry, s_fl);
+      bf_advect<MASKED, NV, H, true>(g, sa, in, out, solid, s_q, s_f);
   __syncthreads();
   float* s_ro = s_q[0];  // ue is dead after phase 3: u of the RD output
   const int tid = threadIdx.x;
