@@ -709,7 +709,7 @@ int yh_adjoint_create(const yh_params* p, int nsteps, int rec_every, int ckpt_ev
   const double tau6[6] = {p->fk.tau_d, p->fk.tau_r, p->fk.tau_si,
                           p->fk.tau_0, p->fk.tau_vp, p->fk.tau_wp};
   adj_k(a, tau6);
-  dim3 bb(64, 4), bg((p->nx + 63) / 64, (p->ny + 3) / 4);
+  const dim3 bg((p->nx + 63) / 64, (p->ny + 3) / 4);  // the 64 x 4 one-step blocks
   a->nblocks = bg.x * bg.y;
   a->nparts = std::max(a->nblocks, ((p->nx + kArTX - 1) / kArTX) * ((p->ny + kArTY - 1) / kArTY));
   hipError_t e = hipSuccess;

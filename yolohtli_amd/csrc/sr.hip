@@ -477,7 +477,6 @@ template <int HALO>
 struct BfTile {
   static constexpr int LW = kBfTX + 2 * HALO, LH = kBfTY + 2 * HALO, LN = LW * LH;
 };
-constexpr int kBfLW = BfTile<kBfHalo>::LW, kBfLH = BfTile<kBfHalo>::LH;
 constexpr int kBfLN = BfTile<kBfHalo>::LN;
 
 // Cells of a thread in the three phases: the phase-1 region (the output
