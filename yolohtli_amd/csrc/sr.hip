@@ -580,9 +580,35 @@ __device__ __forceinline__ long long bf_advect(const SrGeom& g, const SrArgs& sa
   const int nx = g.nx, ny = g.ny;
   const int i0 = blockIdx.x * kBfTX - HALO, j0 = blockIdx.y * kBfTY - HALO;
   const bool commit = blockIdx.x == 0 && blockIdx.y == 0;
-  // 1 the integrals (wave w sums column w, w < 12 of the 16 waves); the state tile and the
-  //   neighbour flags, which do not depend on the solve; in workgroup 0 the
-  //   centre and the tip log slots
+  // 1 the state tile and the neighbour flags, which do not depend on the
+  //   solve; the integrals (wave w sums column w, w < 12 of the 16 waves);
+  //   in workgroup 0 the centre and the tip log slots.  The tile's loads
+  //   are issued first, so they and the partials' loads are in flight
+  //   together.
+  constexpr int IT = (LN + kBfThreads - 1) / kBfThreads;
+  float tv[IT][NV];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int l = tid + it * kBfThreads;
+    if (l < LN) {
+      const int gi = i0 + l % LW, gj = j0 + l / LW;
+      const bool ok = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+      // cells outside the domain hold 0 in every phase: the masked rule's
+      // substitute neighbour can lie there (a hole on one side at the domain
+      // edge), and reads as a hole
+#pragma unroll
+      for (int a = 0; a < NV; ++a) tv[it][a] = ok ? in.f[a][gi + (size_t)nx * gj] : 0.0f;
+    }
+  }
+  // the flags of this thread's cells (the phase-1 region)
+  BfOwn<NV, HALO> o;
+#pragma unroll
+  for (int it = 0; it < IC; ++it) {
+    const int l = tid + it * kBfThreads;
+    const int gi = i0 + 1 + l % W1, gj = j0 + 1 + l / W1;
+    const bool ok = l < N1 && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+    o.fl[it] = ok ? (bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u) : 0u;
+  }
   for (int q = wv; q < 12; q += kBfThreads / 64) {
     const double v = sr_column_sum(sa.partials, sa.nparts, q);
     if (lane == 0) s_I[q] = v;
@@ -599,31 +625,16 @@ __device__ __forceinline__ long long bf_advect(const SrGeom& g, const SrArgs& sa
     s_sc[0] = sp;
     s_sc[1] = cp;
   }
-  constexpr int IT = (LN + kBfThreads - 1) / kBfThreads;
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const int l = tid + it * kBfThreads;
     if (l < LN) {
-      const int gi = i0 + l % LW, gj = j0 + l / LW;
-      const bool ok = gi >= 0 && gi < nx && gj >= 0 && gj < ny;
-      // cells outside the domain hold 0 in every phase: the masked rule's
-      // substitute neighbour can lie there (a hole on one side at the domain
-      // edge), and reads as a hole
 #pragma unroll
       for (int a = 0; a < NV; ++a) {
-        s_q[a][l] = ok ? in.f[a][gi + (size_t)nx * gj] : 0.0f;
+        s_q[a][l] = tv[it][a];
         s_f[a][l] = 0.0f;
       }
     }
-  }
-  // the flags of this thread's cells (the phase-1 region)
-  BfOwn<NV, HALO> o;
-#pragma unroll
-  for (int it = 0; it < IC; ++it) {
-    const int l = tid + it * kBfThreads;
-    const int gi = i0 + 1 + l % W1, gj = j0 + 1 + l / W1;
-    const bool ok = l < N1 && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
-    o.fl[it] = ok ? (bf_flags<MASKED>(gi, gj, nx, ny, solid) | 128u) : 0u;
   }
   if (commit && wv == 0) {
     int cx, cy;
