@@ -477,6 +477,41 @@ def test_autotune_keeps_the_state_and_installs_its_choice(torch_cuda):
         np.testing.assert_array_equal(x, y)
 
 
+def test_autotune_times_whole_calls_of_the_given_size(torch_cuda):
+    """yh_autotune's steps is the call size: each call size is tuned and
+    cached on its own, and a 20-step call on a chosen tiled configuration
+    gives the bits of the same 20 steps on the default one (tiled results do
+    not depend on the shape, the depth or where launches split; the one-step
+    kernel, if chosen, agrees to rounding)."""
+    n = 384
+    p = T.grid_params(n, n, model=T.YH_MODEL_3V_FK, solidSwitch=1)
+    mask = Y.make_holes(n, n, 20, 3.0, 9.0, Y.seed(4, 0, 1))
+    f = Y.init_random(n, n, Y.seed(4, 0, 2), gate=1.0, nvars=3)
+    got = []
+    for steps in (20, 100, 20):
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        cfg, ms = pl.autotune(steps=steps)
+        assert ms > 0 and cfg[0] in (T.YH_KERNEL_DIRECT, T.YH_KERNEL_TILED)
+        pl.step(20)
+        got.append((cfg, pl.get_state()))
+        pl.close()
+    assert got[0][0] == got[2][0]  # the 20-step choice comes from the cache
+    ref = Y.Plan(p)
+    ref.set_mask(mask)
+    ref.set_state(*f)
+    ref.step(20)
+    want = ref.get_state()
+    ref.close()
+    for cfg, st in got:
+        for x, y in zip(st, want):
+            if cfg[0] == T.YH_KERNEL_TILED:
+                np.testing.assert_array_equal(x, y)
+            else:
+                np.testing.assert_allclose(x, y, atol=1e-5)
+
+
 @pytest.mark.parametrize("n,want", [(256, (16, 4, 16)), (512, (16, 4, 16)), (540, (15, 4, 16)),
                                     (1024, (9, 7, 16)), (4096, (8, 8, 8))])
 def test_auto_kernel_follows_the_grid(torch_cuda, n, want):
