@@ -1,7 +1,7 @@
 """The C ABI called from compiled C: tests/c/shim_driver links the
-reference-side shim (yolohtli_amd/shim/yh_shim.c, the binding INTEGRATION.md
-describes for display(), hostPrototypes.h:22-54) against
-include/yolohtli.h and libyolohtli_amd.so.
+reference-side shim (yolohtli_amd/shim/yh_shim.c, the binding a maintainer
+adds to the reference's display() in place of the wrappers of
+hostPrototypes.h:22-54) against include/yolohtli.h and libyolohtli_amd.so.
 
   * host-only calls (defaults, derive, solve_matrix, the parameter CSV round
     trip) against the Python binding and numpy, on the CPU;

@@ -5,7 +5,7 @@ The reference validates a symmetry-reduced run by rebuilding the lab-frame
 tip path from the co-moving tips and phi (DATA/processSymmetry.m:88-89),
   x_lab = -phi_x + x cos(phi_t) + y sin(phi_t)
   y_lab = -phi_y - x sin(phi_t) + y cos(phi_t),
-and comparing it with a plain run (README.md:10-12).  With x = hx*i,
+and comparing it with a plain run (the reference's README.md:10-12).  With x = hx*i,
 y = hy*j measured from cell 0 this is the exact inverse of the frame the
 advection field of Cxy_field_kernel (symmetryReduction.cu:20-62) moves: the
 co-moving state is u_lab(R(-phi_t) x - (phi_x, phi_y)).  Both runs start
