@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -67,13 +70,38 @@ yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
   return c;
 }
 
+// Workgroups of the plan's tiled kernel of this shape that one CU holds at
+// once (its occupancy), cached per instantiation.
+static int tiled_per_cu(const yh_plan* pl, int rows, int waves, int cols, bool prev) {
+  using Key = std::tuple<int, int, int, int, int, int, int>;
+  static std::mutex mu;
+  static std::map<Key, int> cache;
+  const Key key{pl->p.model, pl->p.solidSwitch, pl->k.iso, rows, waves, cols, prev ? 1 : 0};
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  static float dummy;  // selects the previous-u instantiation; never written in a query
+  int n = 0;
+  if (launch_tiled(pl->p.model, pl->p.solidSwitch != 0, rows, waves, cols, 1, pl->k,
+                   CFields{{nullptr, nullptr, nullptr}}, Fields{{nullptr, nullptr, nullptr}},
+                   nullptr, nullptr, nullptr, prev ? &dummy : nullptr, &n) != hipSuccess ||
+      n < 1)
+    n = 1;
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = n;
+  return n;
+}
+
 // Whether a remainder of r < K steps can be merged into the last K-step
 // launch of a call: the shape one row taller per wave (same waves, one
 // column per lane, where a tiled update does not depend on where launches
 // split) is built, with the previous-u output when `prev`, and runs K + r
-// substeps on no more tiles than a K-step launch of the plan's shape, so
-// the merged launch occupies no more CUs and the call takes one launch
-// fewer (a launch's fixed part is about a third of a K=9 launch at 1024^2).
+// substeps in no more rounds of workgroups than a K-step launch of the
+// plan's shape (tiles over the CUs times the workgroups a CU holds at
+// once), so the call takes one launch fewer for no longer a launch (a
+// launch's fixed part is about a third of a K=9 launch at 1024^2).
 static bool merge_ok(const yh_plan* pl, const yh_kernel_cfg& c, int r, bool prev) {
   const int K = c.substeps, rows = c.rows + 1, Km = K + r;
   if (c.cols != 1 || !tiled_supported(rows, c.waves, c.cols)) return false;
@@ -82,8 +110,12 @@ static bool merge_ok(const yh_plan* pl, const yh_kernel_cfg& c, int r, bool prev
   // the reflected halo is exact for at most min(nx, ny) - 1 substeps
   if (Km > 16 || Km > std::min(nx, ny) - 1) return false;
   if (2 * Km >= 64 * c.cols || 2 * Km >= rows * c.waves) return false;
-  return tile_count(nx, ny, rows, c.waves, c.cols, Km) <=
-         tile_count(nx, ny, c.rows, c.waves, c.cols, K);
+  const int64_t cus = std::max(1, pl->cus);
+  const int64_t slots_k = cus * tiled_per_cu(pl, c.rows, c.waves, c.cols, prev);
+  const int64_t slots_m = cus * tiled_per_cu(pl, rows, c.waves, c.cols, prev);
+  const int64_t tk = tile_count(nx, ny, c.rows, c.waves, c.cols, K);
+  const int64_t tm = tile_count(nx, ny, rows, c.waves, c.cols, Km);
+  return (tm + slots_m - 1) / slots_m <= (tk + slots_k - 1) / slots_k;
 }
 
 int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {

@@ -157,7 +157,8 @@ bool tiled_supported(int rows, int waves, int cols) {
   const bool rw = (rows == 4 && (waves == 4 || waves == 8 || waves == 16)) ||
                   (rows == 8 && (waves == 4 || waves == 8 || waves == 16)) ||
                   (rows == 16 && (waves == 4 || waves == 8)) ||
-                  (rows == 7 && waves == 16 && cols == 1);  // 64 x 112: 253 tiles at 1024^2, K=9
+                  (rows == 7 && waves == 16 && cols == 1) ||  // 64 x 112: 253 tiles at 1024^2, K=9
+                  (rows == 5 && waves == 16 && cols == 1);    // 4 x 16 merged with a remainder
   // two columns per lane double the registers: 16 rows, or 8 rows in 16 waves, spill
   return rw && (cols == 1 || (cols == 2 && rows * waves <= 64 && rows <= 8));
 }
@@ -167,16 +168,20 @@ bool tiled_apd_supported(int rows, int waves, int cols) {
   return cols == 1 && (rows == 4 || rows == 8) && (waves == 8 || waves == 16);
 }
 
-// Shapes built with the previous-u output (the APD set and 7 x 16).
+// Shapes built with the previous-u output (the APD set, 7 x 16 and 5 x 16).
 bool tiled_prev_supported(int rows, int waves, int cols) {
-  return tiled_apd_supported(rows, waves, cols) || (cols == 1 && rows == 7 && waves == 16);
+  return tiled_apd_supported(rows, waves, cols) ||
+         (cols == 1 && (rows == 7 || rows == 5) && waves == 16);
 }
 
 template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV = false,
           int ISO = -1>
 static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out,
                                  const uint8_t* codes, hipStream_t s, const ApdState& apd,
-                                 float* prev = nullptr) {
+                                 float* prev = nullptr, int* occ = nullptr) {
+  if (occ)  // the workgroups of this instantiation one CU holds at once; no launch
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        occ, k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV, ISO>, 64 * NW, 0);
   const int wout = 64 * CW - 2 * K, hout = NW * R - 2 * K;
   if (K < 1 || wout <= 0 || hout <= 0) return hipErrorInvalidValue;
   const int tiles_x = (k.nx + wout - 1) / wout;
@@ -190,13 +195,13 @@ static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out
 template <int MODEL, bool MASKED>
 static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KParams& k,
                                  CFields in, Fields out, const uint8_t* codes, hipStream_t s,
-                                 const ApdState* apd, float* prev) {
+                                 const ApdState* apd, float* prev, int* occ) {
   const ApdState a = apd ? *apd : ApdState();
   if (apd) {
     if (prev) return hipErrorInvalidValue;
 #define YH_TILE_APD(RR, WW)                    \
   if (rows == RR && waves == WW && cols == 1) \
-    return launch_tiled_t<MODEL, MASKED, RR, WW, 1, true>(K, k, in, out, codes, s, a);
+    return launch_tiled_t<MODEL, MASKED, RR, WW, 1, true>(K, k, in, out, codes, s, a, nullptr, occ);
     YH_TILE_APD(4, 8)
     YH_TILE_APD(8, 8)
     YH_TILE_APD(4, 16)
@@ -206,36 +211,40 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
   }
   // the shapes the automatic choice takes (7 x 16 and the 8 x 16 its
   // remainders merge into on 1024^2, 8 x 8 on grids with more tiles than CUs,
-  // 4 x 16 on grids with fewer) also come with isotropic diffusion fixed at
-  // compile time
+  // 4 x 16 on grids with fewer, and the 5 x 16 its remainders merge into)
+  // also come with isotropic diffusion fixed at compile time
   if (k.iso && cols == 1) {
 #define YH_TILE_ISO(RR, WW)                                                                     \
   if (rows == RR && waves == WW)                                                                \
     return prev ? launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, true, 1>(K, k, in, out, codes, \
-                                                                          s, a, prev)            \
+                                                                          s, a, prev, occ)       \
                 : launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, false, 1>(K, k, in, out, codes, \
-                                                                           s, a);
+                                                                           s, a, nullptr, occ);
     YH_TILE_ISO(7, 16)
     YH_TILE_ISO(8, 16)
     YH_TILE_ISO(8, 8)
     YH_TILE_ISO(4, 16)
+    YH_TILE_ISO(5, 16)
 #undef YH_TILE_ISO
   }
   if (prev) {
 #define YH_TILE_PREV(RR, WW)                   \
   if (rows == RR && waves == WW && cols == 1) \
-    return launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, true>(K, k, in, out, codes, s, a, prev);
+    return launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, true>(K, k, in, out, codes, s, a, prev, \
+                                                                 occ);
     YH_TILE_PREV(4, 8)
     YH_TILE_PREV(8, 8)
     YH_TILE_PREV(4, 16)
     YH_TILE_PREV(8, 16)
     YH_TILE_PREV(7, 16)
+    YH_TILE_PREV(5, 16)
 #undef YH_TILE_PREV
     return hipErrorInvalidValue;
   }
 #define YH_TILE_CASE(RR, WW, CC)                 \
   if (rows == RR && waves == WW && cols == CC) \
-    return launch_tiled_t<MODEL, MASKED, RR, WW, CC, false>(K, k, in, out, codes, s, a);
+    return launch_tiled_t<MODEL, MASKED, RR, WW, CC, false>(K, k, in, out, codes, s, a, nullptr, \
+                                                            occ);
   YH_TILE_CASE(4, 4, 1)
   YH_TILE_CASE(8, 4, 1)
   YH_TILE_CASE(16, 4, 1)
@@ -245,6 +254,7 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
   YH_TILE_CASE(4, 16, 1)
   YH_TILE_CASE(8, 16, 1)
   YH_TILE_CASE(7, 16, 1)
+  YH_TILE_CASE(5, 16, 1)
   YH_TILE_CASE(4, 4, 2)
   YH_TILE_CASE(8, 4, 2)
   YH_TILE_CASE(4, 8, 2)
@@ -256,13 +266,16 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
 
 hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
                         const KParams& k, CFields in, Fields out, const uint8_t* codes,
-                        hipStream_t s, const ApdState* apd, float* prev) {
+                        hipStream_t s, const ApdState* apd, float* prev, int* occ) {
   if (model == M2V) {
-    return masked ? launch_tiled_m<M2V, true>(rows, waves, cols, K, k, in, out, codes, s, apd, prev)
-                  : launch_tiled_m<M2V, false>(rows, waves, cols, K, k, in, out, codes, s, apd, prev);
+    return masked
+               ? launch_tiled_m<M2V, true>(rows, waves, cols, K, k, in, out, codes, s, apd, prev, occ)
+               : launch_tiled_m<M2V, false>(rows, waves, cols, K, k, in, out, codes, s, apd, prev,
+                                            occ);
   }
-  return masked ? launch_tiled_m<MFK, true>(rows, waves, cols, K, k, in, out, codes, s, apd, prev)
-                : launch_tiled_m<MFK, false>(rows, waves, cols, K, k, in, out, codes, s, apd, prev);
+  return masked
+             ? launch_tiled_m<MFK, true>(rows, waves, cols, K, k, in, out, codes, s, apd, prev, occ)
+             : launch_tiled_m<MFK, false>(rows, waves, cols, K, k, in, out, codes, s, apd, prev, occ);
 }
 
 // -------------------------------------------------------------------- tips

@@ -324,9 +324,12 @@ hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields o
                                const uint8_t* codes, hipStream_t s,
                                const ApdState* apd = nullptr);
 // prev (optional): u of the output cells after K-1 substeps, for the tip test.
+// occ (optional): instead of launching, the workgroups of the instantiation
+// this call would launch that one CU holds at once.
 hipError_t launch_tiled(int model, bool masked, int rows, int waves, int cols, int K,
                         const KParams& k, CFields in, Fields out, const uint8_t* codes,
-                        hipStream_t s, const ApdState* apd = nullptr, float* prev = nullptr);
+                        hipStream_t s, const ApdState* apd = nullptr, float* prev = nullptr,
+                        int* occ = nullptr);
 bool tiled_supported(int rows, int waves, int cols);
 bool tiled_apd_supported(int rows, int waves, int cols);
 bool tiled_prev_supported(int rows, int waves, int cols);
