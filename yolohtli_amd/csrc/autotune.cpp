@@ -25,7 +25,7 @@ std::vector<yh_kernel_cfg> candidates(const yh_plan* pl) {
   // 4-wave regions and deeper fusion; large ones the 8- and 16-wave shapes
   const int Ks[] = {1, 2, 4, 6, 8, 9, 10, 12, 14, 16};
   const int shapes[][3] = {{4, 4, 1}, {8, 4, 1}, {16, 4, 1}, {4, 8, 1}, {8, 8, 1},
-                           {4, 16, 1}, {8, 16, 1}, {16, 8, 1}, {7, 16, 1},
+                           {4, 16, 1}, {5, 16, 1}, {8, 16, 1}, {16, 8, 1}, {7, 16, 1},
                            {4, 8, 2}, {8, 8, 2}, {4, 16, 2}};
   for (int K : Ks) {
     if (K > kmax) continue;
