@@ -93,9 +93,13 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
     return hip_fail(he, "yh_autotune");
   }
   const int64_t count0 = pl->count;
-  const bool keep0 = pl->keep_prev, prev0 = pl->have_prev;
+  const bool prev0 = pl->have_prev;
   const yh_kernel_cfg cfg0 = pl->cfg;
-  pl->keep_prev = false;
+  // The candidates run calls as the plan will, keeping u of the step before
+  // the last when the plan does (the tip test's input), so their timing
+  // counts that output, and the kernels the chosen configuration launches
+  // (the previous-u and merged-remainder variants) are loaded and their
+  // occupancy known before the caller's first call.
   const bool apd0 = pl->apd_on;  // the timed steps must not touch the APD state
   std::vector<yh_kernel_cfg> cands = candidates(pl);
   pl->apd_on = false;
@@ -129,10 +133,10 @@ extern "C" int yh_autotune(yh_plan* pl, int steps, void* stream, yh_kernel_cfg* 
   (void)hipStreamSynchronize(s);
   for (int a = 0; a < pl->nv; ++a) (void)hipFree(save[a]);
   pl->count = count0;
-  pl->keep_prev = keep0;
   pl->apd_on = apd0;
   (void)prev0;
-  pl->have_prev = false;  // buf[cur^1] holds a candidate's output, not the previous step
+  pl->have_prev = false;  // buf[cur^1] and prev_u hold a candidate's output
+  pl->past_prev_u = false;
   if (rc) {
     pl->cfg = cfg0;
     return rc;
