@@ -513,11 +513,12 @@ def test_autotune_times_whole_calls_of_the_given_size(torch_cuda):
 
 
 @pytest.mark.parametrize("n,want", [(256, (16, 4, 16)), (512, (16, 4, 16)), (540, (15, 4, 16)),
-                                    (1024, (9, 7, 16)), (4096, (8, 8, 8))])
+                                    (1024, (8, 4, 16)), (4096, (8, 8, 8))])
 def test_auto_kernel_follows_the_grid(torch_cuda, n, want):
-    """Without timing, AUTO takes 64 x 112 regions at K=9 when their tiles
-    fill between half and all of the 256 CUs (1024^2: 253), 8 x 8 waves at
-    K=8 when there are more tiles than CUs (4096^2), else 4 x 16 at the
+    """Without timing, AUTO takes 4 x 16 at K=8 where 64 x 112 regions at
+    K=9 would fill between half and all of the 256 CUs and two 4 x 16
+    workgroups fit on a CU (1024^2: 484 tiles on 512 places), 8 x 8 waves
+    at K=8 when there are more tiles than CUs (4096^2), else 4 x 16 at the
     largest K <= 16 whose tiles number at most 256 (512^2: 16 x 16 tiles of
     32 x 32 outputs; 540^2: K=15, 16 x 16 tiles of 34 x 34)."""
     pl = Y.Plan(T.grid_params(n, n, model=T.YH_MODEL_3V_FK))

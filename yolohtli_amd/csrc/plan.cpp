@@ -22,20 +22,25 @@ static int64_t tile_count(int nx, int ny, int rows, int waves, int cols, int K) 
   return (int64_t)((nx + wout - 1) / wout) * ((ny + hout - 1) / hout);
 }
 
-// YH_KERNEL_AUTO without timing anything (yh_autotune times the candidates):
-// 64 x 112 regions at K=9 when their tiles fill between half and all of the
-// CUs, one each (the best shape measured at 1024^2); 64 x 64 regions at K=8
-// when there are more tiles than CUs, so two workgroups per CU overlap one's
-// loads and stores with the other's substeps (the best measured at 4096^2);
-// else 64 x 64 regions in 16 waves of 4 rows at K=8.
-yh_kernel_cfg auto_cfg(int nx, int ny, int cus) {
+// YH_KERNEL_AUTO without timing anything (yh_autotune times the candidates).
+// When 64 x 112 regions at K=9 would fill between half and all of the CUs,
+// one each: 64 x 64 regions in 16 waves of 4 rows at K=8 if two of them fit
+// on a CU at once (`pairs`) and their tiles fit in one such round (1024^2:
+// 484 tiles, the best measured there, with remainders merged into 5 x 16),
+// else the 64 x 112 regions.  64 x 64 regions in 8 waves at K=8 when there
+// are more tiles than that, so two workgroups per CU overlap one's loads and
+// stores with the other's substeps (the best measured at 4096^2); else 64 x
+// 64 regions in 16 waves of 4 rows, as deep as one round allows.
+yh_kernel_cfg auto_cfg(int nx, int ny, int cus, bool pairs) {
   const int64_t t716 = tile_count(nx, ny, 7, 16, 1, 9);
   const int64_t ncu = cus > 0 ? cus : 256;
   yh_kernel_cfg c;
   if (t716 > ncu)
     c = yh_kernel_cfg{YH_KERNEL_TILED, 8, 8, 8, 1};
   else if (2 * t716 >= ncu)
-    c = yh_kernel_cfg{YH_KERNEL_TILED, 9, 7, 16, 1};
+    c = pairs && tile_count(nx, ny, 4, 16, 1, 8) <= 2 * ncu
+            ? yh_kernel_cfg{YH_KERNEL_TILED, 8, 4, 16, 1}
+            : yh_kernel_cfg{YH_KERNEL_TILED, 9, 7, 16, 1};
   else {
     // a grid too small to fill half the CUs with 64 x 112 regions: 64 x 64
     // regions at the deepest fusion whose tiles still fit on the CUs at one
@@ -52,21 +57,6 @@ yh_kernel_cfg auto_cfg(int nx, int ny, int cus) {
   }
   // the reflected halo is exact for at most min(nx, ny) - 1 substeps
   c.substeps = std::max(1, std::min(c.substeps, std::min(nx, ny) - 1));
-  return c;
-}
-
-yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
-  yh_kernel_cfg c = pl->cfg;
-  if (c.variant == YH_KERNEL_AUTO) c = auto_cfg(pl->p.nx, pl->p.ny, pl->cus);
-  if (c.variant == YH_KERNEL_TILED) {
-    const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
-    c.substeps = std::max(1, std::min(c.substeps, kmax));
-    if (pl->apd_on && !tiled_apd_supported(c.rows, c.waves, c.cols)) {
-      c.rows = 4;  // the fused APD update is built for these shapes only
-      c.waves = 16;
-      c.cols = 1;
-    }
-  }
   return c;
 }
 
@@ -92,6 +82,22 @@ static int tiled_per_cu(const yh_plan* pl, int rows, int waves, int cols, bool p
   std::lock_guard<std::mutex> g(mu);
   cache[key] = n;
   return n;
+}
+
+yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
+  yh_kernel_cfg c = pl->cfg;
+  if (c.variant == YH_KERNEL_AUTO)
+    c = auto_cfg(pl->p.nx, pl->p.ny, pl->cus, tiled_per_cu(pl, 4, 16, 1, false) >= 2);
+  if (c.variant == YH_KERNEL_TILED) {
+    const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
+    c.substeps = std::max(1, std::min(c.substeps, kmax));
+    if (pl->apd_on && !tiled_apd_supported(c.rows, c.waves, c.cols)) {
+      c.rows = 4;  // the fused APD update is built for these shapes only
+      c.waves = 16;
+      c.cols = 1;
+    }
+  }
+  return c;
 }
 
 // Whether a remainder of r < K steps can be merged into the last K-step

@@ -51,8 +51,9 @@ namespace yh {
 // the mouse stimulus on).
 inline bool plan_general(const yh_plan* pl) { return !standard_branch(pl->p) || pl->stim_on; }
 // The YH_KERNEL_AUTO rule for an nx x ny grid on a device of `cus` compute
-// units (a tiled shape and its substeps, capped at min(nx, ny) - 1).
-yh_kernel_cfg auto_cfg(int nx, int ny, int cus);
+// units (a tiled shape and its substeps, capped at min(nx, ny) - 1); pairs:
+// a CU holds two workgroups of the 4 x 16 shape at once.
+yh_kernel_cfg auto_cfg(int nx, int ny, int cus, bool pairs = false);
 // Resolve YH_KERNEL_AUTO to a concrete configuration for this plan.
 yh_kernel_cfg resolve_cfg(const yh_plan* pl);
 // Advance `nsteps` on the plan's buffers, keeping u of the step before the
