@@ -34,7 +34,8 @@ extern "C" {
 /* At startup, after parameterSetup(), domainObjects() and initGates():
  * creates the plan (replacing the allocation and cudaMemcpyToSymbol of
  * main.cu:213-402), the mask and the state.  The plan keeps
- * YH_KERNEL_AUTO; a caller may add yh_autotune(plan, 400, ...) here. */
+ * YH_KERNEL_AUTO; a caller may add yh_autotune(plan, sampleIt, ...) here
+ * (tuned for calls of sampleIt steps, the plain branch's call size). */
 int yh_shim_init(const paramVar* param, const bool* solid_h, stateVar g_h);
 /* The plain branch of display() (main.cu:876-888): sampleIt steps, then the
  * tip test (main.cu:961-964) when plotTip. */
