@@ -337,7 +337,9 @@ def test_tiled_results_do_not_depend_on_launch_split(torch_cuda, model):
     in the order of the cells they mirror, so they stay bitwise mirrors and a
     cell's value after n steps does not depend on the fused depth, the tile
     shape or where calls split: masked runs of 100 steps with K = 3, 5, 8, 9
-    on four shapes, and in calls of 37 + 63, agree bit for bit.  The previous
+    on five shapes, and in calls of 37 + 63 (where remainders merge into
+    the shape one row taller: 7 x 16 -> 8 x 16, 4 x 16 -> 5 x 16), agree
+    bit for bit.  The previous
     state is not kept, so every step runs in the tiled kernel (a shape built
     without the fused previous-u output would end each call with the
     one-step kernel, whose stencil sums in another order)."""
@@ -347,7 +349,9 @@ def test_tiled_results_do_not_depend_on_launch_split(torch_cuda, model):
     f = Y.init_random(n, n, Y.seed(5, 0, 2), gate=1.0, nvars=T.nvars(p))
     runs = []
     for cfg, calls in (((3, 8, 4, 1), [100]), ((5, 4, 16, 1), [100]), ((8, 8, 8, 1), [100]),
-                       ((9, 7, 16, 1), [100]), ((9, 7, 16, 1), [37, 63])):
+                       ((9, 7, 16, 1), [100]), ((9, 7, 16, 1), [37, 63]),
+                       ((8, 4, 16, 1), [100]), ((8, 4, 16, 1), [37, 63]),
+                       ((12, 5, 16, 1), [100])):
         pl = Y.Plan(p)
         pl.set_mask(mask)
         pl.set_state(*f)
@@ -371,7 +375,8 @@ def test_random_grids_shapes_and_splits_agree(torch_cuda):
     another order) except where FK's sharp Heaviside flips on a threshold
     that u meets to within FP32 rounding (at most 1 cell in 1000)."""
     rng = np.random.default_rng(20261202)
-    shapes = [(4, 4), (8, 4), (16, 4), (4, 8), (8, 8), (16, 8), (4, 16), (8, 16), (7, 16)]
+    shapes = [(4, 4), (8, 4), (16, 4), (4, 8), (8, 8), (16, 8), (4, 16), (5, 16), (8, 16),
+              (7, 16)]
     for case in range(24):
         nx, ny = (int(x) for x in rng.integers(5, 401, size=2))
         model = MODELS[int(rng.integers(2))]
