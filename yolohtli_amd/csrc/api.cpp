@@ -73,6 +73,7 @@ void make_kparams(const yh_params& p, const yh_derived& d, KParams* k) {
   k->fk_C2 = (float)(tcdt / f.tau_wm);
   k->hw = (float)p.heaviside_width;
   k->iso = k->trx == k->try_ ? 1 : 0;
+  k->wt = 0;
 }
 
 // splitmix64 (Steele, Lea & Flood 2014).

@@ -124,6 +124,20 @@ static bool merge_ok(const yh_plan* pl, const yh_kernel_cfg& c, int r, bool prev
   return (tm + slots_m - 1) / slots_m <= (tk + slots_k - 1) / slots_k;
 }
 
+// Whether a tiled launch of this shape writes its outputs through to memory
+// (the isotropic kernels come in both forms): when it places about two
+// workgroups on each CU (1.5 x the CUs or more, and the kernel fits two per
+// CU).  Dirty lines left in each XCD's L2 are written back at the end of the
+// launch, when the CUs are idle; written through, that traffic overlaps the
+// second workgroup's loads (4x16 K=8 at 1024^2: 17.0 -> 15.7 us a launch).
+// A launch of one workgroup per CU ran 3% slower written through (512^2), so
+// it keeps plain stores.
+static int write_through(const yh_plan* pl, int rows, int waves, int cols, int K, bool prev) {
+  const int64_t cus = std::max(1, pl->cus);
+  const int64_t t = tile_count(pl->p.nx, pl->p.ny, rows, waves, cols, K);
+  return 2 * t >= 3 * cus && tiled_per_cu(pl, rows, waves, cols, prev) >= 2;
+}
+
 int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
   if (nsteps < 0) return YH_EINVAL;
   if (nsteps == 0) return YH_OK;
@@ -183,7 +197,9 @@ int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
       fields(pl->cur, &ci, &co);
       apd.count0 = (int)done;
       float* prev = (prev_fused && K == remaining) ? pl->prev_u : nullptr;
-      YH_HIP(launch_tiled(pl->p.model, masked, rows, c.waves, c.cols, K, pl->k, ci, co,
+      KParams kl = pl->k;
+      kl.wt = !ap && write_through(pl, rows, c.waves, c.cols, K, prev != nullptr);
+      YH_HIP(launch_tiled(pl->p.model, masked, rows, c.waves, c.cols, K, kl, ci, co,
                           masked ? pl->codes_d : nullptr, s, ap, prev));
       pl->cur ^= 1;
       remaining -= K;

@@ -479,20 +479,11 @@ struct BfTile {
 };
 constexpr int kBfLN = BfTile<kBfHalo>::LN;
 
-// A store of one float of field `base` (element idx < 2^30) written
-// through to memory (the sc1 policy: coherent at device scope) instead of
-// left dirty in this XCD's L2: the next kernel of the step reads these
-// fields from workgroups on every XCD, so a dirty line would only be
-// written back at the end of the launch, on the critical path between the
-// two kernels of a step (a step measured 14.9 against 16.0 us at 512^2).
-// The descriptor covers 4 GiB from base (raw buffer, 32-bit elements: the
-// third-dword format gfx9 descriptors use), so the unsigned byte offset of
-// any element of a 2^30-cell grid is in range.
-__device__ __forceinline__ void store_through(float* base, uint32_t idx, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(
-      __float_as_uint(v), __builtin_amdgcn_make_buffer_rsrc(base, 0, -1, 0x00020000),
-      (int)(idx * 4u), 0, 16 /* sc1 */);
-}
+// The new state and the RD output are written through to memory
+// (store_through): the next kernel of the step reads these fields from
+// workgroups on every XCD, so dirty lines would only be written back at the
+// end of the launch, between the two kernels of a step (a step measured
+// 14.9 against 16.0 us at 512^2).
 
 // Cells of a thread in the three phases: the phase-1 region (the output
 // tile grown by HALO - 1 cells), cell l = tid + it * kBfThreads of it, so a

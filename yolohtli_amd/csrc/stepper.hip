@@ -141,7 +141,8 @@ hipError_t launch_direct_codes(int model, const KParams& k, CFields in, Fields o
 }
 
 // ------------------------------------------------------------------- tiled
-template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV, int ISO = -1>
+template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV, int ISO = -1,
+          bool WT = false>
 __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, Fields out,
                                                    const uint8_t* __restrict__ codes, int K,
                                                    int tiles_x, int ntiles, const ApdState apd,
@@ -149,8 +150,9 @@ __global__ __launch_bounds__(64 * NW) void k_tiled(const KParams k, CFields in, 
   __shared__ TileLds<MODEL, NW, CW> lds;
   const int tile = xcd_tile(blockIdx.x, ntiles);
   const int ty = tile / tiles_x;
-  tile_advance<MODEL, MASKED, R, NW, CW, APD, PREV, ISO>(k, in, out, codes, K,
-                                                         tile - ty * tiles_x, ty, lds, apd, prev);
+  tile_advance<MODEL, MASKED, R, NW, CW, APD, PREV, ISO, WT>(k, in, out, codes, K,
+                                                             tile - ty * tiles_x, ty, lds, apd,
+                                                             prev);
 }
 
 bool tiled_supported(int rows, int waves, int cols) {
@@ -175,19 +177,19 @@ bool tiled_prev_supported(int rows, int waves, int cols) {
 }
 
 template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD, bool PREV = false,
-          int ISO = -1>
+          int ISO = -1, bool WT = false>
 static hipError_t launch_tiled_t(int K, const KParams& k, CFields in, Fields out,
                                  const uint8_t* codes, hipStream_t s, const ApdState& apd,
                                  float* prev = nullptr, int* occ = nullptr) {
   if (occ)  // the workgroups of this instantiation one CU holds at once; no launch
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        occ, k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV, ISO>, 64 * NW, 0);
+        occ, k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV, ISO, WT>, 64 * NW, 0);
   const int wout = 64 * CW - 2 * K, hout = NW * R - 2 * K;
   if (K < 1 || wout <= 0 || hout <= 0) return hipErrorInvalidValue;
   const int tiles_x = (k.nx + wout - 1) / wout;
   const int tiles_y = (k.ny + hout - 1) / hout;
   const int ntiles = tiles_x * tiles_y;
-  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV, ISO>), dim3(ntiles),
+  hipLaunchKernelGGL((k_tiled<MODEL, MASKED, R, NW, CW, APD, PREV, ISO, WT>), dim3(ntiles),
                      dim3(64 * NW), 0, s, k, in, out, codes, K, tiles_x, ntiles, apd, prev);
   return hipGetLastError();
 }
@@ -212,20 +214,26 @@ static hipError_t launch_tiled_m(int rows, int waves, int cols, int K, const KPa
   // the shapes the automatic choice takes (7 x 16 and the 8 x 16 its
   // remainders merge into on 1024^2, 8 x 8 on grids with more tiles than CUs,
   // 4 x 16 on grids with fewer, and the 5 x 16 its remainders merge into)
-  // also come with isotropic diffusion fixed at compile time
+  // also come with isotropic diffusion fixed at compile time, and with
+  // write-through stores (k.wt) for launches of two workgroups per CU
   if (k.iso && cols == 1) {
-#define YH_TILE_ISO(RR, WW)                                                                     \
-  if (rows == RR && waves == WW)                                                                \
-    return prev ? launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, true, 1>(K, k, in, out, codes, \
-                                                                          s, a, prev, occ)       \
-                : launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, false, 1>(K, k, in, out, codes, \
-                                                                           s, a, nullptr, occ);
+#define YH_TILE_ISO_W(RR, WW, W)                                              \
+  return prev ? launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, true, 1, W>(   \
+                    K, k, in, out, codes, s, a, prev, occ)                    \
+              : launch_tiled_t<MODEL, MASKED, RR, WW, 1, false, false, 1, W>(  \
+                    K, k, in, out, codes, s, a, nullptr, occ);
+#define YH_TILE_ISO(RR, WW)               \
+  if (rows == RR && waves == WW) {        \
+    if (k.wt) YH_TILE_ISO_W(RR, WW, true) \
+    YH_TILE_ISO_W(RR, WW, false)          \
+  }
     YH_TILE_ISO(7, 16)
     YH_TILE_ISO(8, 16)
     YH_TILE_ISO(8, 8)
     YH_TILE_ISO(4, 16)
     YH_TILE_ISO(5, 16)
 #undef YH_TILE_ISO
+#undef YH_TILE_ISO_W
   }
   if (prev) {
 #define YH_TILE_PREV(RR, WW)                   \

@@ -53,8 +53,12 @@ struct TileLds {
 // K >= 1 (the launchers reject K < 1).
 // ISO: -1 takes k.iso at run time; 1 (0) fixes an isotropic (anisotropic)
 // diffusion at compile time, which drops that branch from the row loop.
+// WT: the new state is written through to memory (store_through) rather
+// than left dirty in the XCD's L2, for launches that hold two workgroups on
+// a CU: the write-back then overlaps the other workgroup's loads instead of
+// delaying the end of the launch.
 template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false, bool PREV = false,
-          int ISO = -1>
+          int ISO = -1, bool WT = false>
 __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Fields out,
                                              const uint8_t* __restrict__ codes, int K, int tx,
                                              int ty, TileLds<MODEL, NW, CW>& lds,
@@ -275,8 +279,10 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
               if (PREV) *at32(prev_u, idx) = h ? 0.0f : u_old;
               if (!APD) {
 #pragma unroll
-                for (int a = 0; a < NV; ++a)
-                  *at32(out.f[a], idx) = h ? 0.0f : X[a][r][c];
+                for (int a = 0; a < NV; ++a) {
+                  if (WT) store_through(out.f[a], idx, h ? 0.0f : X[a][r][c]);
+                  else *at32(out.f[a], idx) = h ? 0.0f : X[a][r][c];
+                }
               }
             }
           }

@@ -37,7 +37,21 @@ struct KParams {
   float fk_C1, fk_C2;                // tcdt/tau_wp, tcdt/tau_wm
   float hw;                          // tanh-smoothed Heaviside width (gradient path)
   int iso;                           // trx == try: one stencil weight for both axes
+  int wt;                            // host side: the launcher picks the write-through kernel
 };
+
+// A store of one float of field `base` (element idx < 2^30) written through
+// to memory (the sc1 policy: coherent at device scope) instead of left dirty
+// in this XCD's L2, where it would be written back at the end of the launch,
+// on the critical path before the next launch.  The descriptor covers 4 GiB
+// from base (raw buffer, 32-bit elements: the third-dword format gfx9
+// descriptors use), so the unsigned byte offset of any element of a
+// 2^30-cell grid is in range.
+__device__ __forceinline__ void store_through(float* base, uint32_t idx, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(
+      __float_as_uint(v), __builtin_amdgcn_make_buffer_rsrc(base, 0, -1, 0x00020000),
+      (int)(idx * 4u), 0, 16 /* sc1 */);
+}
 
 // Fields of the state (u, v, w); unused entries are null.
 struct Fields {
