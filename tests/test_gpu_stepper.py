@@ -366,6 +366,36 @@ def test_tiled_results_do_not_depend_on_launch_split(torch_cuda, model):
             np.testing.assert_array_equal(x, y)
 
 
+@pytest.mark.parametrize("model", MODELS)
+def test_write_through_launches_equal_plain_ones(torch_cuda, model):
+    """At 1024^2, 4 x 16 launches of K = 8 (418 tiles: two workgroups on most
+    CUs, all resident) and the 5 x 16 launches their remainders merge into
+    write the state through to memory; at K = 16 (1024 tiles, several
+    rounds) the same shape stores plainly.  A store's cache policy changes no
+    value: 96 masked steps in the three forms, with and without the kept
+    previous state, agree bit for bit."""
+    n = 1024
+    p = T.grid_params(n, n, model=model, solidSwitch=1)
+    mask = Y.make_holes(n, n, 60, 3.0, 12.0, Y.seed(6, 0, 1))
+    f = Y.init_random(n, n, Y.seed(6, 0, 2), gate=1.0, nvars=T.nvars(p))
+    runs = []
+    for K, calls, keep in ((16, [96], 0), (8, [96], 0), (8, [37, 59], 0), (8, [37, 59], 1),
+                           (16, [37, 59], 1)):
+        print(f"write-through case K={K} calls={calls} keep_prev={keep}", flush=True)
+        pl = Y.Plan(p)
+        pl.set_mask(mask)
+        pl.set_state(*f)
+        pl.set_kernel(T.YH_KERNEL_TILED, K, 4, 16, 1)
+        pl.set_option(T.YH_OPT_KEEP_PREV, keep)
+        for c in calls:
+            pl.step(c)
+        runs.append(pl.get_state())
+        pl.close()
+    for r in runs[1:]:
+        for x, y in zip(runs[0], r):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_random_grids_shapes_and_splits_agree(torch_cuda):
     """24 seeded cases: grid sides 5..400 (ragged against every tile), model,
     mask and hole count, a one-column shape, K and a split of the steps into

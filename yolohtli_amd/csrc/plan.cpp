@@ -125,17 +125,18 @@ static bool merge_ok(const yh_plan* pl, const yh_kernel_cfg& c, int r, bool prev
 }
 
 // Whether a tiled launch of this shape writes its outputs through to memory
-// (the isotropic kernels come in both forms): when it places about two
-// workgroups on each CU (1.5 x the CUs or more, and the kernel fits two per
-// CU).  Dirty lines left in each XCD's L2 are written back at the end of the
-// launch, when the CUs are idle; written through, that traffic overlaps the
-// second workgroup's loads (4x16 K=8 at 1024^2: 17.0 -> 15.7 us a launch).
-// A launch of one workgroup per CU ran 3% slower written through (512^2), so
-// it keeps plain stores.
+// (the isotropic kernels come in both forms): when all its workgroups are
+// resident at once and most CUs hold two (1.5 x the CUs or more).  Dirty
+// lines left in each XCD's L2 are written back at the end of the launch,
+// when the CUs are idle; written through, that traffic overlaps the second
+// workgroup's loads (4x16 K=8 at 1024^2: 16.9 -> 15.6 us a launch).  A
+// launch of one workgroup per CU ran 3% slower written through (512^2), one
+// of many rounds of workgroups no faster (4096^2) and the batched kernel
+// over 50 M cells 5% slower, so those keep plain stores.
 static int write_through(const yh_plan* pl, int rows, int waves, int cols, int K, bool prev) {
   const int64_t cus = std::max(1, pl->cus);
   const int64_t t = tile_count(pl->p.nx, pl->p.ny, rows, waves, cols, K);
-  return 2 * t >= 3 * cus && tiled_per_cu(pl, rows, waves, cols, prev) >= 2;
+  return 2 * t >= 3 * cus && t <= cus * tiled_per_cu(pl, rows, waves, cols, prev);
 }
 
 int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {
