@@ -23,7 +23,7 @@ std::vector<yh_kernel_cfg> candidates(const yh_plan* pl) {
   const int kmax = std::min(pl->p.nx, pl->p.ny) - 1;
   // small grids (512^2 has 169 tiles of 64 x 64 on 256 CUs) favour
   // 4-wave regions and deeper fusion; large ones the 8- and 16-wave shapes
-  const int Ks[] = {1, 2, 4, 6, 8, 9, 10, 12, 14, 16};
+  const int Ks[] = {1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 16};
   const int shapes[][3] = {{4, 4, 1}, {8, 4, 1}, {16, 4, 1}, {4, 8, 1}, {8, 8, 1},
                            {4, 16, 1}, {5, 16, 1}, {8, 16, 1}, {16, 8, 1}, {7, 16, 1},
                            {4, 8, 2}, {8, 8, 2}, {4, 16, 2}};
