@@ -91,9 +91,13 @@ int yh_step_noalloc(const yh_params* p, float* const out[3], const float* const 
   if (masked) YH_HIP(launch_codes(solid, codes, p->nx, p->ny, s));
   KParams k;
   make_kparams(*p, d, &k);
-  // the kernel a plan left on YH_KERNEL_AUTO runs; the result does not
-  // depend on where launches split, so it equals yh_step's bit for bit
-  const yh_kernel_cfg c = auto_cfg(p->nx, p->ny, cus);
+  // the kernel a plan left on YH_KERNEL_AUTO runs, with the plan's store
+  // policy per launch (neither the remainder merge nor the store policy
+  // changes a value: the result does not depend on where launches split,
+  // so it equals yh_step's bit for bit)
+  const bool pairs = tiled_per_cu(p->model, masked, k, 4, 16, 1, false) >= 2;
+  const yh_kernel_cfg c = auto_cfg(p->nx, p->ny, cus, pairs);
+  const int per_cu = tiled_per_cu(p->model, masked, k, c.rows, c.waves, c.cols, false);
   const int K = c.substeps;
   const int launches = (nsteps + K - 1) / K;
   // launch i writes `out` when launches - 1 - i is even, else the scratch
@@ -105,7 +109,10 @@ int yh_step_noalloc(const yh_params* p, float* const out[3], const float* const 
     const int steps = std::min(K, remaining);
     CFields ci{{src[0], src[1], src[2]}};
     Fields co{{dst[0], dst[1], nv == 3 ? dst[2] : nullptr}};
-    YH_HIP(launch_tiled(p->model, masked, c.rows, c.waves, c.cols, steps, k, ci, co,
+    KParams kl = k;
+    kl.wt = tiled_write_through(
+        tiled_tile_count(p->nx, p->ny, c.rows, c.waves, c.cols, steps), cus, per_cu);
+    YH_HIP(launch_tiled(p->model, masked, c.rows, c.waves, c.cols, steps, kl, ci, co,
                         masked ? codes : nullptr, s));
     for (int a = 0; a < 3; ++a) src[a] = co.f[a];
     remaining -= steps;

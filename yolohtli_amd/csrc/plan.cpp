@@ -3,9 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <map>
 #include <mutex>
-#include <tuple>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -15,11 +13,14 @@
 
 namespace yh {
 
-// Tiles of one launch of K substeps on a rows x waves x cols shape.
-static int64_t tile_count(int nx, int ny, int rows, int waves, int cols, int K) {
+int64_t tiled_tile_count(int nx, int ny, int rows, int waves, int cols, int K) {
   const int wout = 64 * cols - 2 * K, hout = rows * waves - 2 * K;
   if (wout <= 0 || hout <= 0) return INT64_MAX;
   return (int64_t)((nx + wout - 1) / wout) * ((ny + hout - 1) / hout);
+}
+
+static int64_t tile_count(int nx, int ny, int rows, int waves, int cols, int K) {
+  return tiled_tile_count(nx, ny, rows, waves, cols, K);
 }
 
 // YH_KERNEL_AUTO without timing anything (yh_autotune times the candidates).
@@ -60,28 +61,43 @@ yh_kernel_cfg auto_cfg(int nx, int ny, int cus, bool pairs) {
   return c;
 }
 
-// Workgroups of the plan's tiled kernel of this shape that one CU holds at
-// once (its occupancy), cached per instantiation.
-static int tiled_per_cu(const yh_plan* pl, int rows, int waves, int cols, bool prev) {
-  using Key = std::tuple<int, int, int, int, int, int, int>;
+// Workgroups of the tiled kernel of this shape (model, mask, the isotropy
+// of k, with or without the previous-u output) that one CU holds at once
+// (its occupancy), cached per instantiation in a fixed table, so neither
+// the query nor the cache allocates (the no-allocation entry point uses it).
+int tiled_per_cu(int model, bool masked, const KParams& k, int rows, int waves, int cols,
+                 bool prev) {
+  struct Entry {
+    int key[7];
+    int n;
+  };
   static std::mutex mu;
-  static std::map<Key, int> cache;
-  const Key key{pl->p.model, pl->p.solidSwitch, pl->k.iso, rows, waves, cols, prev ? 1 : 0};
+  static Entry table[256];
+  static int used = 0;
+  const int key[7] = {model, masked ? 1 : 0, k.iso, rows, waves, cols, prev ? 1 : 0};
   {
     std::lock_guard<std::mutex> g(mu);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
+    for (int e = 0; e < used; ++e)
+      if (std::equal(key, key + 7, table[e].key)) return table[e].n;
   }
   static float dummy;  // selects the previous-u instantiation; never written in a query
   int n = 0;
-  if (launch_tiled(pl->p.model, pl->p.solidSwitch != 0, rows, waves, cols, 1, pl->k,
-                   CFields{{nullptr, nullptr, nullptr}}, Fields{{nullptr, nullptr, nullptr}},
-                   nullptr, nullptr, nullptr, prev ? &dummy : nullptr, &n) != hipSuccess ||
+  if (launch_tiled(model, masked, rows, waves, cols, 1, k, CFields{{nullptr, nullptr, nullptr}},
+                   Fields{{nullptr, nullptr, nullptr}}, nullptr, nullptr, nullptr,
+                   prev ? &dummy : nullptr, &n) != hipSuccess ||
       n < 1)
     n = 1;
   std::lock_guard<std::mutex> g(mu);
-  cache[key] = n;
+  if (used < 256) {  // every instantiation fits; past that, query again
+    std::copy(key, key + 7, table[used].key);
+    table[used].n = n;
+    ++used;
+  }
   return n;
+}
+
+static int tiled_per_cu(const yh_plan* pl, int rows, int waves, int cols, bool prev) {
+  return tiled_per_cu(pl->p.model, pl->p.solidSwitch != 0, pl->k, rows, waves, cols, prev);
 }
 
 yh_kernel_cfg resolve_cfg(const yh_plan* pl) {
@@ -124,19 +140,24 @@ static bool merge_ok(const yh_plan* pl, const yh_kernel_cfg& c, int r, bool prev
   return (tm + slots_m - 1) / slots_m <= (tk + slots_k - 1) / slots_k;
 }
 
-// Whether a tiled launch of this shape writes its outputs through to memory
-// (the isotropic kernels come in both forms): when all its workgroups are
-// resident at once and most CUs hold two (1.5 x the CUs or more).  Dirty
-// lines left in each XCD's L2 are written back at the end of the launch,
-// when the CUs are idle; written through, that traffic overlaps the second
-// workgroup's loads (4x16 K=8 at 1024^2: 16.9 -> 15.6 us a launch).  A
-// launch of one workgroup per CU ran 3% slower written through (512^2), one
-// of many rounds of workgroups no faster (4096^2) and the batched kernel
-// over 50 M cells 5% slower, so those keep plain stores.
+// Whether a tiled launch of `tiles` workgroups, `per_cu` of which fit on a
+// CU, writes its outputs through to memory (the isotropic kernels come in
+// both forms): when all its workgroups are resident at once and most CUs
+// hold two (1.5 x the CUs or more).  Dirty lines left in each XCD's L2 are
+// written back at the end of the launch, when the CUs are idle; written
+// through, that traffic overlaps the second workgroup's loads (4x16 K=8 at
+// 1024^2: 16.9 -> 15.6 us a launch).  A launch of one workgroup per CU ran
+// 3% slower written through (512^2), one of many rounds of workgroups no
+// faster (4096^2) and the batched kernel over 50 M cells 5% slower, so
+// those keep plain stores.
+bool tiled_write_through(int64_t tiles, int cus, int per_cu) {
+  const int64_t c = std::max(1, cus);
+  return 2 * tiles >= 3 * c && tiles <= c * per_cu;
+}
+
 static int write_through(const yh_plan* pl, int rows, int waves, int cols, int K, bool prev) {
-  const int64_t cus = std::max(1, pl->cus);
-  const int64_t t = tile_count(pl->p.nx, pl->p.ny, rows, waves, cols, K);
-  return 2 * t >= 3 * cus && t <= cus * tiled_per_cu(pl, rows, waves, cols, prev);
+  return tiled_write_through(tile_count(pl->p.nx, pl->p.ny, rows, waves, cols, K), pl->cus,
+                             tiled_per_cu(pl, rows, waves, cols, prev));
 }
 
 int plan_advance(yh_plan* pl, int nsteps, hipStream_t s) {

@@ -54,6 +54,15 @@ inline bool plan_general(const yh_plan* pl) { return !standard_branch(pl->p) || 
 // units (a tiled shape and its substeps, capped at min(nx, ny) - 1); pairs:
 // a CU holds two workgroups of the 4 x 16 shape at once.
 yh_kernel_cfg auto_cfg(int nx, int ny, int cus, bool pairs = false);
+// Workgroups of the tiled kernel of this shape one CU holds at once
+// (cached; allocates nothing).
+int tiled_per_cu(int model, bool masked, const KParams& k, int rows, int waves, int cols,
+                 bool prev);
+// Tiles of one launch of K substeps on a rows x waves x cols shape.
+int64_t tiled_tile_count(int nx, int ny, int rows, int waves, int cols, int K);
+// Whether a tiled launch of `tiles` workgroups, `per_cu` per CU at most,
+// writes its outputs through to memory (KParams::wt).
+bool tiled_write_through(int64_t tiles, int cus, int per_cu);
 // Resolve YH_KERNEL_AUTO to a concrete configuration for this plan.
 yh_kernel_cfg resolve_cfg(const yh_plan* pl);
 // Advance `nsteps` on the plan's buffers, keeping u of the step before the
