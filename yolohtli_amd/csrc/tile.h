@@ -53,10 +53,10 @@ struct TileLds {
 // K >= 1 (the launchers reject K < 1).
 // ISO: -1 takes k.iso at run time; 1 (0) fixes an isotropic (anisotropic)
 // diffusion at compile time, which drops that branch from the row loop.
-// WT: the new state is written through to memory (store_through) rather
-// than left dirty in the XCD's L2, for launches that hold two workgroups on
-// a CU: the write-back then overlaps the other workgroup's loads instead of
-// delaying the end of the launch.
+// WT: the new state (and prev_u) is written through to memory
+// (store_through) rather than left dirty in the XCD's L2, for launches that
+// hold two workgroups on a CU: the write-back then overlaps the other
+// workgroup's loads instead of delaying the end of the launch.
 template <int MODEL, bool MASKED, int R, int NW, int CW, bool APD = false, bool PREV = false,
           int ISO = -1, bool WT = false>
 __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Fields out,
@@ -276,7 +276,10 @@ __device__ __forceinline__ void tile_advance(const KParams& k, CFields in, Field
             if (lc >= K && lc < WR - K && px < nx && py < ny) {
               const bool h = MASKED && ((hole[c] >> r) & 1u);
               const uint32_t idx = (uint32_t)py * (uint32_t)nx + (uint32_t)px;
-              if (PREV) *at32(prev_u, idx) = h ? 0.0f : u_old;
+              if (PREV) {
+                if (WT) store_through(prev_u, idx, h ? 0.0f : u_old);
+                else *at32(prev_u, idx) = h ? 0.0f : u_old;
+              }
               if (!APD) {
 #pragma unroll
                 for (int a = 0; a < NV; ++a) {
